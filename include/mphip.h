@@ -1,0 +1,309 @@
+/*
+ * mphip.h -- C-ABI of libmphip, the MI355X (gfx950) backend for MagnetoPhotonic.jl's
+ * coupled 3-D time step.
+ *
+ * Every entry point replaces one launcher of the reference's kernel layer
+ * (src/fdtd/Kernels.jl `_ka_*` functions, the per-backend override points) or one
+ * backend hook of src/core/Backend.jl.  The reference file:line each one replaces is
+ * given beside it.  Conventions:
+ *
+ *  - Plain pointers and sizes only.  Device pointers are `void*` obtained from
+ *    mp_malloc (or any HIP allocation on the current device).
+ *  - Volumes are column-major with x fastest, as Julia arrays are: cell (i,j,k),
+ *    0-based, lives at i + nx*(j + ny*k).  The reference is 1-based; every index
+ *    crossing this ABI is 0-based.
+ *  - Cell lists use int32 linear indices (grids up to 2^31 cells); the reference
+ *    keeps Int64 on the host.
+ *  - Composite Julia isbits types map to C structs with identical field order:
+ *    DLPole{T} -> {T C1, C2, C3}; FerrimagnetParameters{Float64} -> mp_ferri_params.
+ *  - Every entry returns 0 on success and a negative code on failure; the message is
+ *    available from mp_last_error() (thread-local).  No C++ exception crosses the ABI.
+ *  - All launches are asynchronous on the library stream (mp_set_stream); nothing
+ *    synchronises unless the call says so.  The library never keeps caller pointers
+ *    past a call.  One host thread drives a stream; calls are not re-entrant per stream.
+ */
+#ifndef MPHIP_H
+#define MPHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- dtypes ---------------------------------------------------------------------- */
+enum mp_dtype {
+    MP_F32 = 0,   /* Float32 */
+    MP_F64 = 1,   /* Float64 */
+    MP_BF16 = 2,  /* bfloat16 storage (FP32 arithmetic) */
+    MP_F16 = 3    /* IEEE half storage with a power-of-two scale (FP32 arithmetic) */
+};
+
+/* Field component codes: src/fdtd/Kernels.jl:1-7 (_component_code). */
+enum mp_component { MP_EX = 1, MP_EY = 2, MP_EZ = 3, MP_HX = 4, MP_HY = 5, MP_HZ = 6 };
+
+enum mp_status {
+    MP_OK = 0,
+    MP_ERR_ARG = -1,      /* bad argument (ArgumentError / DimensionMismatch in the reference) */
+    MP_ERR_HIP = -2,      /* HIP runtime error */
+    MP_ERR_UNSUPPORTED = -3
+};
+
+/* ---- data model -------------------------------------------------------------------- */
+
+/* Yee volumes and 1-D metric arrays of an FDTDState (src/fdtd/Solver.jl:9-53,78-94;
+ * src/fdtd/Fields.jl:1-11).  inv_eps_* are ABSOLUTE 1/(eps0*eps_r) (Solver.jl:87-94),
+ * stored in field_dtype; inv_d_* are the CT copies of Axis1D.inv_d_cell/inv_d_dual
+ * (Solver.jl:81-86, src/grid/Grid.jl:24-38). */
+typedef struct mp_yee {
+    int64_t nx, ny, nz;
+    int32_t field_dtype;    /* storage of E, H, D, inv_eps: MP_F32 or MP_F64 */
+    int32_t compute_dtype;  /* arithmetic type CT: MP_F32 or MP_F64 */
+    void *ex, *ey, *ez, *hx, *hy, *hz, *dx, *dy, *dz;
+    const void *inv_eps_x, *inv_eps_y, *inv_eps_z;
+    const void *inv_d_cell_x, *inv_d_cell_y, *inv_d_cell_z;
+    const void *inv_d_dual_x, *inv_d_dual_y, *inv_d_dual_z;
+} mp_yee;
+
+/* CFS-CPML state (src/fdtd/CPML.jl:79-189).  Per-axis coefficient vectors inv_kappa,
+ * a, b (length n of that axis, CT) and the 24 convolution-memory slabs: an x-slab is
+ * (w, ny, nz), a y-slab (nx, w, nz), a z-slab (nx, ny, w) (CPML.jl:183-188).
+ * psi_h / psi_d order: xy_lo, xy_hi, xz_lo, xz_hi, yz_lo, yz_hi, yx_lo, yx_hi,
+ * zx_lo, zx_hi, zy_lo, zy_hi (psi_<component><derivative axis>_<side>, CPML.jl:133-146).
+ * npml_* = 0 disables that axis (CPML.jl:151 _cpml_width).  For MP_BF16 / MP_F16
+ * storage the slab holds psi * psi_scale. */
+typedef struct mp_cpml {
+    int32_t npml_x, npml_y, npml_z;
+    int32_t psi_dtype;
+    const void *inv_kappa[3], *a[3], *b[3];
+    void *psi_h[12];
+    void *psi_d[12];
+    double psi_scale;
+} mp_cpml;
+
+/* One component's Drude-Lorentz ADE list (src/fdtd/Dispersion.jl:93-113;
+ * Solver.jl:125-169).  P, J are (n, npoles) column-major: P[m + p*n]. */
+typedef struct mp_ade {
+    int64_t n;
+    int32_t npoles;
+    int32_t dtype;        /* CT: type of E_old, fill, inv_eps, poles and the arithmetic */
+    int32_t state_dtype;  /* storage of P and J: CT, or MP_BF16 / MP_F16 (mixed form) */
+    int32_t _pad;
+    double state_scale;   /* stored = value * state_scale for BF16/F16 storage */
+    void *P, *J, *E_old;
+    const int32_t *idx;   /* 0-based linear cell index */
+    const void *fill, *inv_eps;   /* per active cell, CT; inv_eps = 1/(eps0*eps) */
+    const void *poles;            /* npoles x {C1, C2, C3}, CT (DLPole{CT}) */
+} mp_ade;
+
+/* Magneto-optic gyration ADE state (src/fdtd/MagnetoOptic.jl:1-39).  The reference
+ * hard-wires the pair (a, b) = (Ey, Ez) driven by m_x (Kernels.jl:233-305); here the
+ * pair is whatever two E volumes the caller passes.  mpos maps list position -> film
+ * cell (-1 = none). */
+typedef struct mp_mo {
+    int64_t n;
+    int32_t npoles;
+    int32_t dtype;  /* CT */
+    void *Pa_from_b, *Ja_from_b, *Pb_from_a, *Jb_from_a;  /* (n, npoles) */
+    void *E_old_a, *E_old_b;
+    const int32_t *idx;
+    const void *fill;
+    const int32_t *mpos;
+    const void *poles;
+} mp_mo;
+
+/* Film (material-cell) state: ThermalState, MagnetizationState, AbsorptionState and
+ * the region maps (src/physics/Thermal.jl:5-61, src/physics/Magnetization.jl:265-284,
+ * Solver.jl:114-159).  pos_* map a film cell to its position in each ADE list (-1 =
+ * inactive there), the reference's active_axis_position_map (ReferenceRaster.jl:218). */
+typedef struct mp_film {
+    int64_t n;
+    int32_t dtype;  /* CT: storage of Te, Tl, Ts_*, m_* */
+    int32_t _pad;
+    const int32_t *cells;
+    const int32_t *pos_x, *pos_y, *pos_z;
+    void *Te, *Tl, *Ts_TM, *Ts_RE;
+    void *m_TM_x, *m_TM_y, *m_TM_z, *m_RE_x, *m_RE_y, *m_RE_z;
+    double *U_abs, *window_sum, *peak;   /* always FP64 */
+} mp_film;
+
+/* FerrimagnetParameters{T} field order (src/physics/Models.jl:5-103), as FP64. */
+typedef struct mp_ferri_params {
+    double gamma_e, Cl, Cs_TM, Cs_RE, Gel, Ges_TM, Ges_RE, Gamma_E;
+    double Hsw0, Tsw, dTsw, gamma_gyro, Ms_TM, Ms_RE, mu_TM, mu_RE;
+    double alpha0_TM, alpha0_RE, J_TMRE, J_TMTM, J_RERE, K_TM, K_RE;
+    double T_Curie, T_comp, T0, TcT, TcR, inv_TcT, inv_TcR;
+    double beta_T, beta_R, floor_T, floor_R, val300_T, val300_R, inv_val300_T, inv_val300_R;
+    double GammaT0, GammaT_hot, GammaR0, GammaR_hot, Tsw_long, dTsw_long, inv_dTsw_long;
+    double ellT, ellR, inv_ellT, inv_ellR, GammaEx, eta_quench, inv_eta_quench;
+    double GammaT_tail, eta_tail, inv_eta_tail, mTM_tail_target, field_to_K;
+    double GTlT, GTlR, GsTR, S_TM, S_RE, Q_voigt_TM, Q_voigt_RE;
+    double omega_pump, eps_imag_pump, eps_real_pump, omega_probe, eps_imag_probe, eps_real_probe;
+    double kb, moment_ratio_TM_RE, inv_Ms_sum;
+    double inv_T_Curie, inv_Cl, inv_Cs_TM, inv_Cs_RE;
+    double ferri_RE_scale, re_exchange_lag, inv_mu_TM, inv_mu_RE;
+    double inv_Ms_TM, inv_Ms_RE, twelve_J_TMTM, twelve_J_RERE;
+    double two_K_over_Ms_TM, two_K_over_Ms_RE;
+    double inv_tau_TM_demag, inv_tau_RE_demag, moment_ratio_RE_TM;
+    double HK_T_field, HK_R_field, HAF_T_field, HAF_R_field, alphaT_hot, alphaR_hot, seed_tilt;
+} mp_ferri_params;
+
+/* Brillouin m_eq lookup table (src/physics/Magnetization.jl:50-86), stored in CT. */
+typedef struct mp_lut {
+    const void *tm, *re;
+    double T_min, inv_dT;
+    int32_t n;
+    int32_t dtype;
+} mp_lut;
+
+/* One DFT / flux monitor plane (src/sim/ProbeReadout.jl:11-170, generalised from the
+ * reference's two x-normal planes to any axis).  The tangential pair (a, b) is the
+ * cyclic pair after the normal axis: x -> (y, z), y -> (z, x), z -> (x, y), so the
+ * signed forward flux is Ea*Hb - Eb*Ha (x: Ey*Hz - Ez*Hy, Kernels.jl:447).
+ * dft holds four ComplexF64 arrays (Ea, Eb, Ha, Hb), each (n1, n2, nb) column-major
+ * with interleaved re/im; energy is (n1, n2); n1, n2 are the plane's two in-plane
+ * sizes in axis order (x-plane: ny, nz). */
+typedef struct mp_plane {
+    int32_t axis;    /* 0 = x, 1 = y, 2 = z (plane normal) */
+    int32_t _pad;
+    int64_t index;   /* 0-based plane index along the normal axis */
+    double *dft_ea, *dft_eb, *dft_ha, *dft_hb;
+    double *energy;
+} mp_plane;
+
+#define MP_MAX_PLANES 8
+
+/* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */
+
+/* Select the device and create the library state.  Backend.jl:19-33 (__init__ /
+ * _GPU_FUNCTIONAL) and ext/MagnetoPhotonicCUDAExt.jl:328-331. */
+int mp_init(int device);
+/* Thread-local message of the last failing call ("" if none). */
+const char *mp_last_error(void);
+/* zeros_backend / free_device! (Backend.jl:37-48,121-126). */
+int mp_malloc(void **ptr, size_t bytes);
+int mp_free(void *ptr);
+/* adapt_backend / to_host (Backend.jl:49-60); synchronous copies. */
+int mp_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int mp_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int mp_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* Byte fill on the library stream (fill! of zeros). */
+int mp_memset(void *ptr, int value, size_t bytes);
+/* Typed fill on the library stream: fill_backend (Backend.jl:37-48). */
+int mp_fill(void *ptr, int32_t dtype, int64_t n, double value);
+/* synchronize(backend) (Backend.jl:85-95). */
+int mp_synchronize(void);
+/* device_memory_info (Backend.jl:106-119). */
+int mp_mem_info(size_t *free_bytes, size_t *total_bytes);
+/* Launch stream for every later call (a hipStream_t, or NULL for the null stream). */
+int mp_set_stream(void *stream);
+void *mp_get_stream(void);
+
+/* Event timing on the library stream (replaces step_profiling!, Solver.jl:195-252). */
+int mp_event_create(void **event);
+int mp_event_destroy(void *event);
+int mp_event_record(void *event);
+int mp_event_elapsed_ms(void *start, void *stop, float *ms);  /* synchronises on stop */
+
+/* Per-kernel timing: when on, each launch is bracketed by events; totals are read with
+ * mp_kernel_stats (names are the kernel names below).  Off by default. */
+int mp_profile_enable(int on);
+int mp_kernel_stats(const char *name, int64_t *launches, double *total_ms);
+
+/* ---- hot-path kernels ---------------------------------------------------------------- */
+
+/* Yee H update with CFS-CPML psi_H: H -= dt/mu0 * curl E over i<nx-1, j<ny-1, k<nz-1.
+ * Replaces _ka_update_H_3d! (src/fdtd/Kernels.jl:522-541; kernel :60-101).
+ * dt_mu0 is formed as CT(dt)/CT(mu0) as the launcher does (Kernels.jl:539). */
+int mp_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0);
+
+/* Yee E update with CFS-CPML psi_D: D += dt*curl H, E = D*inv_eps over i,j,k >= 1.
+ * Replaces _ka_update_E_3d! (Kernels.jl:543-565; kernel :103-152). */
+int mp_update_E_3d(const mp_yee *g, mp_cpml *c, double dt);
+
+/* Soft point source at one cell: E += v, D += v/inv_eps (or H += v), in CT.
+ * Replaces _ka_inject_soft_3d! (Kernels.jl:567-576; kernel :154-174). */
+int mp_inject_soft_3d(const mp_yee *g, int32_t component, int64_t i, int64_t j, int64_t k,
+                      double value);
+
+/* x-normal mode-source plane: E_c += v*profile, D_c += .../inv_eps, H -/+= hcorr*amp.
+ * profile is (ny, nz) in CT.  Replaces _ka_inject_mode_x_3d! (Kernels.jl:578-593;
+ * kernel :176-192). */
+int mp_inject_mode_x_3d(const mp_yee *g, int32_t component, int64_t i, const void *profile,
+                        double value, double hcorr);
+
+/* Plane source on any axis with the two-cell linear interpolation weights of
+ * _axis_weights (src/fdtd/Source.jl:74-86).  Arithmetic follows the host loop
+ * inject!(::FieldState, ::Grid3D, ::PlaneSource) (Source.jl:188-206, 216-236): the
+ * increment w*value and D += w*value/inv_eps are formed in FP64 and rounded once into
+ * the stored field.  The reference has no device kernel for this (it is a host loop);
+ * pass i1 = -1 for a single-cell plane. */
+int mp_inject_plane_3d(const mp_yee *g, int32_t component, int32_t axis, int64_t i0, double w0,
+                       int64_t i1, double w1, double value);
+
+/* Semi-implicit Drude-Lorentz ADE patch on one component's active list.
+ * Replaces _ka_patch_E_dispersive! (Kernels.jl:595-604; kernel :194-231).
+ * E is that component's field volume; inv_alpha is formed as CT(2)/CT(dt). */
+int mp_patch_E_dispersive(void *E, mp_ade *s, int32_t field_dtype, double dt);
+
+/* Crank-Nicolson gyrotropic exchange between Ea and Eb on the MO list.
+ * Replaces _ka_patch_E_mo_gyration! (Kernels.jl:606-619; kernel :233-305).
+ * m_TM / m_RE are the film magnetisation components that drive the gyration (m_x in the
+ * reference).  inv_eps_a / inv_eps_b are the full volumes (field dtype). */
+int mp_patch_E_mo_gyration(void *Ea, void *Eb, mp_mo *s, const void *inv_eps_a,
+                           const void *inv_eps_b, int32_t field_dtype, const void *m_TM,
+                           const void *m_RE, double Q_voigt_TM, double Q_voigt_RE, double dt);
+
+/* Per-EM-step absorbed power: pabs = max(E.sum(J),0) (ade_work) or pabs_factor*|E|^2;
+ * U_abs, window_sum += pabs*dt; peak = max.  FP64.  ax/ay/az may be NULL when
+ * has_ade_work == 0.  Replaces _ka_pabs_accumulate! (Kernels.jl:623-646; :313-355). */
+int mp_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
+                       const mp_ade *az, int32_t has_ade_work, double pabs_factor, double dt);
+
+/* 4TM + two-sublattice LLB, `subcycles` substeps of dt/subcycles, FP64 arithmetic.
+ * has_pabs = 1 consumes window_sum/dt and resets it; 0 is the cool-down form
+ * (relax_step!).  dt, T_min, inv_dT are rounded to the film CT first, as the launcher
+ * does.  p holds FerrimagnetParameters{CT} widened to FP64.
+ * Replaces _ka_multiphysics_step! (Kernels.jl:648-666; kernel :362-412) with
+ * update_4tm (src/physics/Thermal.jl:24-40) and llb_step (Magnetization.jl:105-263). */
+int mp_multiphysics_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut, double dt,
+                         int32_t subcycles, int32_t brillouin_iters, int32_t has_pabs);
+
+/* Per-step DFT of the tangential fields on up to MP_MAX_PLANES planes: the nb phase
+ * factors (cos wt*dt, -sin wt*dt) are formed once into phase (device, 2*nb doubles),
+ * then every plane cell accumulates 4 complex values per bin plus the signed flux
+ * integral.  omega_bins is a device array of nb doubles.  Replaces
+ * _ka_probe_dft_accumulate! (Kernels.jl:671-689; kernels :420-461). */
+int mp_probe_dft_accumulate(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
+                            const double *omega_bins, double *phase, int32_t nb, double t,
+                            double dt);
+
+/* Mode-weighted plane sums of Ea and Eb on two planes of one axis, reduced on device:
+ * out[0..3] = sum(Ea_pre*w), sum(Eb_pre*w), sum(Ea_post*w), sum(Eb_post*w) with
+ * w = mode_w*area (n1*n2 doubles each, device).  out is a device array of 4 doubles.
+ * Replaces _ka_probe_trace_plane! + the host sum (Kernels.jl:691-699, :467-478;
+ * ProbeReadout.jl:286-298). */
+int mp_probe_trace_plane(const mp_yee *g, int32_t axis, int64_t i_pre, int64_t i_post,
+                         const double *mode_w, const double *area, double *out);
+
+/* Plane-integrated Poynting flux sum((Ea*Hb - Eb*Ha)*area) on one plane, written to
+ * out[slot] (device).  Replaces the host loop of record!(::FluxMonitor)
+ * (src/sim/Monitors.jl:263-341) with an on-device reduction. */
+int mp_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const double *area,
+                  double *out, int64_t slot);
+
+/* Film reductions on device (src/sim/Monitors.jl:487-578 FilmAverage/SwitchedFraction,
+ * src/physics/Magnetization.jl:280-283 norms): out[0..9] = means of m_TM_xyz, m_RE_xyz,
+ * Te, Tl, then the fraction of cells with m_TM_x < 0, then max Te.  out: 10 device
+ * doubles. */
+int mp_film_reduce(const mp_film *f, double *out);
+
+/* Non-finite scan of a device array (NaNGuard, Monitors.jl:580-606): *flag (device
+ * int32) is set to 1 if any element is NaN or Inf; the caller clears it. */
+int mp_any_nonfinite(const void *x, int32_t dtype, int64_t n, int32_t *flag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPHIP_H */

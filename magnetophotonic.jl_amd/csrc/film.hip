@@ -1,0 +1,285 @@
+// List kernels over the dispersive / film cells: Drude-Lorentz ADE patch, magneto-optic
+// gyration and absorbed-power accumulation.
+//
+// Reference: ade_patch_kernel! (src/fdtd/Kernels.jl:194-231), mo_gyration_kernel!
+// (:233-305), pabs_accumulate_kernel! (:313-355) and their launchers (:595-646).
+// One thread per list entry; P/J are (n, npoles) column-major so a wave reads 64
+// consecutive entries of one pole.  E is gathered through the cell index list.
+#include "common.h"
+
+namespace mp {
+
+// P/J storage: CT, or BF16/F16 holding value*scale (FP32 arithmetic).
+template <typename C, typename T> struct StateIO {
+    static __device__ __forceinline__ C load(const T *p, int64_t o, float) { return (C)p[o]; }
+    static __device__ __forceinline__ void store(T *p, int64_t o, C v, float) { p[o] = (T)v; }
+};
+template <> struct StateIO<float, bf16> {
+    static __device__ __forceinline__ float load(const bf16 *p, int64_t o, float isc) { return ld<float>(p[o]) * isc; }
+    static __device__ __forceinline__ void store(bf16 *p, int64_t o, float v, float sc) { p[o] = st<bf16>(v * sc); }
+};
+template <> struct StateIO<float, f16s> {
+    static __device__ __forceinline__ float load(const f16s *p, int64_t o, float isc) { return ld<float>(p[o]) * isc; }
+    static __device__ __forceinline__ void store(f16s *p, int64_t o, float v, float sc) { p[o] = st<f16s>(v * sc); }
+};
+
+constexpr int MAXP = 8;
+
+template <typename S, typename C, typename T>
+__global__ __launch_bounds__(256) void ade_patch_kernel(
+    S *__restrict__ E, T *__restrict__ P, T *__restrict__ J, C *__restrict__ E_old,
+    const int32_t *__restrict__ idxs, const C *__restrict__ fill, const C *__restrict__ inv_eps,
+    const C *__restrict__ poles, int np, int64_t N, C inv_alpha, float sc, float isc) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int64_t idx = idxs[n];
+    const C Eprev = E_old[n];
+    const C f = fill[n];
+    const C ie = inv_eps[n];
+    C Pold[MAXP], Jold[MAXP];
+    C sum_psi = (C)0, sum_beta = (C)0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+        if (p >= np) break;
+        Pold[p] = StateIO<C, T>::load(P, n + p * N, isc);
+        Jold[p] = StateIO<C, T>::load(J, n + p * N, isc);
+        const C beta = poles[3 * p + 2] * f;
+        const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
+        sum_psi += psi;
+        sum_beta += beta;
+    }
+    const C den = (C)1 + sum_beta * ie;
+    const C Etemp = (C)E[idx];
+    if (finiteT(Etemp) && finiteT(sum_psi) && finiteT(den) && __builtin_fabs((double)den) >= (double)(C)1e-30) {
+        const C Efinal = (Etemp - sum_psi * ie) / den;
+        E[idx] = (S)Efinal;
+        E_old[n] = Efinal;
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+            if (p >= np) break;
+            const C beta = poles[3 * p + 2] * f;
+            const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
+            const C Pnew = psi + beta * Efinal;
+            StateIO<C, T>::store(P, n + p * N, Pnew, sc);
+            StateIO<C, T>::store(J, n + p * N, (Pnew - Pold[p]) * inv_alpha - Jold[p], sc);
+        }
+    }
+}
+
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void mo_gyration_kernel(
+    S *__restrict__ Ea, S *__restrict__ Eb, C *__restrict__ Pa, C *__restrict__ Ja,
+    C *__restrict__ Pb, C *__restrict__ Jb, C *__restrict__ Eoa, C *__restrict__ Eob,
+    const int32_t *__restrict__ idxs, const C *__restrict__ fill, const int32_t *__restrict__ mposv,
+    const S *__restrict__ iea, const S *__restrict__ ieb, const C *__restrict__ m_TM,
+    const C *__restrict__ m_RE, const C *__restrict__ poles, int np, int64_t N, C Q_TM, C Q_RE,
+    C inv_alpha) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int64_t idx = idxs[n];
+    const int32_t mpos = mposv[n];
+    if (mpos < 0) return;
+    const C f = fill[n];
+    const C qeff = Q_TM * m_TM[mpos] + Q_RE * m_RE[mpos];
+    const C Ea_drive = (C)Ea[idx];
+    const C Eb_drive = (C)Eb[idx];
+    const C Ea_old = Eoa[n];
+    const C Eb_old = Eob[n];
+    C sum_psi_a = (C)0, sum_psi_b = (C)0, sumC3 = (C)0;
+    for (int p = 0; p < np; ++p) {
+        const C C1 = poles[3 * p], C2 = poles[3 * p + 1], C3 = poles[3 * p + 2];
+        const C beta = C3 * f * qeff;
+        sum_psi_a += fmaT<C>(C1, Pa[n + p * N], fmaT<C>(C2, Ja[n + p * N], beta * Eb_old));
+        sum_psi_b += fmaT<C>(C1, Pb[n + p * N], fmaT<C>(C2, Jb[n + p * N], beta * Ea_old));
+        sumC3 += C3;
+    }
+    const C ica = (C)iea[idx];
+    const C icb = (C)ieb[idx];
+    const C Bmo = f * qeff * sumC3;
+    const C half = (C)0.5;
+    const C aa = half * ica * Bmo;
+    const C ab = half * icb * Bmo;
+    const C RHSa = Ea_drive - ica * sum_psi_a - aa * Eb_drive;
+    const C RHSb = Eb_drive + icb * sum_psi_b + ab * Ea_drive;
+    const C den = (C)1 + aa * ab;
+    const C Eb_final = (RHSb + ab * RHSa) / den;
+    const C Ea_final = RHSa - aa * Eb_final;
+    if (finiteT(Ea_final) && finiteT(Eb_final)) {
+        Ea[idx] = (S)Ea_final;
+        Eb[idx] = (S)Eb_final;
+        Eoa[n] = Ea_final;
+        Eob[n] = Eb_final;
+        const C Ea_mid = half * (Ea_drive + Ea_final);
+        const C Eb_mid = half * (Eb_drive + Eb_final);
+        for (int p = 0; p < np; ++p) {
+            const C C1 = poles[3 * p], C2 = poles[3 * p + 1], C3 = poles[3 * p + 2];
+            const C beta = C3 * f * qeff;
+            const C Pa_old = Pa[n + p * N], Ja_old = Ja[n + p * N];
+            const C Pa_new = fmaT<C>(C1, Pa_old, fmaT<C>(C2, Ja_old, beta * Eb_old)) + beta * Eb_mid;
+            Pa[n + p * N] = Pa_new;
+            Ja[n + p * N] = (Pa_new - Pa_old) * inv_alpha - Ja_old;
+            const C Pb_old = Pb[n + p * N], Jb_old = Jb[n + p * N];
+            const C Pb_new = fmaT<C>(C1, Pb_old, fmaT<C>(C2, Jb_old, beta * Ea_old)) + beta * Ea_mid;
+            Pb[n + p * N] = Pb_new;
+            Jb[n + p * N] = (Pb_new - Pb_old) * inv_alpha - Jb_old;
+        }
+    }
+}
+
+template <typename S, typename C, typename T, bool ADE_WORK>
+__global__ __launch_bounds__(256) void pabs_kernel(
+    double *__restrict__ U_abs, double *__restrict__ window, double *__restrict__ peak,
+    const S *__restrict__ Ex, const S *__restrict__ Ey, const S *__restrict__ Ez,
+    const int32_t *__restrict__ cells, const int32_t *__restrict__ pos_x,
+    const int32_t *__restrict__ pos_y, const int32_t *__restrict__ pos_z,
+    const T *__restrict__ Jx, const T *__restrict__ Jy, const T *__restrict__ Jz,
+    int64_t nJx, int64_t nJy, int64_t nJz, int np, double pabs_factor, double dt, int64_t N,
+    float isc) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int64_t li = cells[n];
+    const double ex = (double)Ex[li], ey = (double)Ey[li], ez = (double)Ez[li];
+    double pabs;
+    if (ADE_WORK) {
+        double jx = 0.0, jy = 0.0, jz = 0.0;
+        const int32_t px = pos_x[n], py = pos_y[n], pz = pos_z[n];
+        for (int p = 0; p < np; ++p) {
+            if (px >= 0) jx += (double)StateIO<C, T>::load(Jx, px + p * nJx, isc);
+            if (py >= 0) jy += (double)StateIO<C, T>::load(Jy, py + p * nJy, isc);
+            if (pz >= 0) jz += (double)StateIO<C, T>::load(Jz, pz + p * nJz, isc);
+        }
+        pabs = jl_max(ex * jx + ey * jy + ez * jz, 0.0);
+    } else {
+        pabs = pabs_factor * (ex * ex + ey * ey + ez * ez);
+    }
+    const double pdt = pabs * dt;
+    U_abs[n] += pdt;
+    window[n] += pdt;
+    if (pabs > peak[n]) peak[n] = pabs;
+}
+
+template <typename S, typename C, typename T>
+static int launch_ade(void *E, mp_ade *s, double dt) {
+    const int64_t N = s->n;
+    const double sc = (s->state_dtype == MP_BF16 || s->state_dtype == MP_F16) && s->state_scale > 0 ? s->state_scale : 1.0;
+    const C inv_alpha = (C)2 / (C)dt;
+    KernelTimer t("ade_patch");
+    ade_patch_kernel<S, C, T><<<cdiv(N, 256), 256, 0, stream()>>>(
+        (S *)E, (T *)s->P, (T *)s->J, (C *)s->E_old, s->idx, (const C *)s->fill,
+        (const C *)s->inv_eps, (const C *)s->poles, s->npoles, N, inv_alpha, (float)sc,
+        (float)(1.0 / sc));
+    MP_LAUNCH_CHECK("ade_patch_kernel");
+    return MP_OK;
+}
+
+template <typename S, typename C, typename T>
+static int launch_pabs(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
+                       const mp_ade *az, int has_ade_work, double pabs_factor, double dt) {
+    const int64_t N = f->n;
+    KernelTimer t("pabs");
+    if (has_ade_work) {
+        const double sc = (ax->state_dtype == MP_BF16 || ax->state_dtype == MP_F16) && ax->state_scale > 0 ? ax->state_scale : 1.0;
+        pabs_kernel<S, C, T, true><<<cdiv(N, 256), 256, 0, stream()>>>(
+            f->U_abs, f->window_sum, f->peak, (const S *)g->ex, (const S *)g->ey, (const S *)g->ez,
+            f->cells, f->pos_x, f->pos_y, f->pos_z, (const T *)ax->J, (const T *)ay->J,
+            (const T *)az->J, ax->n, ay->n, az->n, ax->npoles, pabs_factor, dt, N, (float)(1.0 / sc));
+    } else {
+        pabs_kernel<S, C, T, false><<<cdiv(N, 256), 256, 0, stream()>>>(
+            f->U_abs, f->window_sum, f->peak, (const S *)g->ex, (const S *)g->ey, (const S *)g->ez,
+            f->cells, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, 0,
+            pabs_factor, dt, N, 1.0f);
+    }
+    MP_LAUNCH_CHECK("pabs_kernel");
+    return MP_OK;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+MP_API int mp_patch_E_dispersive(void *E, mp_ade *s, int32_t field_dtype, double dt) {
+    if (!s) return fail_arg("mp_patch_E_dispersive: null state");
+    if (s->n <= 0 || s->npoles <= 0) return MP_OK;  // Kernels.jl:599
+    if (s->npoles > MAXP) return fail_arg("mp_patch_E_dispersive: at most %d poles", MAXP);
+    if (!E || !s->P || !s->J || !s->E_old || !s->idx || !s->fill || !s->inv_eps || !s->poles)
+        return fail_arg("mp_patch_E_dispersive: null array");
+    const int32_t sd = s->state_dtype;
+    if (field_dtype == MP_F32 && s->dtype == MP_F32) {
+        if (sd == MP_F32) return launch_ade<float, float, float>(E, s, dt);
+        if (sd == MP_BF16) return launch_ade<float, float, bf16>(E, s, dt);
+        if (sd == MP_F16) return launch_ade<float, float, f16s>(E, s, dt);
+    } else if (field_dtype == MP_F64 && s->dtype == MP_F64 && sd == MP_F64) {
+        return launch_ade<double, double, double>(E, s, dt);
+    } else if (field_dtype == MP_F32 && s->dtype == MP_F64 && sd == MP_F64) {
+        return launch_ade<float, double, double>(E, s, dt);
+    }
+    return fail_arg("mp_patch_E_dispersive: unsupported dtypes field=%d ct=%d state=%d", field_dtype,
+                    s->dtype, sd);
+}
+
+MP_API int mp_patch_E_mo_gyration(void *Ea, void *Eb, mp_mo *s, const void *iea, const void *ieb,
+                                  int32_t field_dtype, const void *m_TM, const void *m_RE,
+                                  double Q_TM, double Q_RE, double dt) {
+    if (!s) return fail_arg("mp_patch_E_mo_gyration: null state");
+    if (s->n <= 0 || s->npoles <= 0) return MP_OK;  // Kernels.jl:612
+    if (!Ea || !Eb || !iea || !ieb || !m_TM || !m_RE || !s->idx || !s->mpos || !s->fill || !s->poles)
+        return fail_arg("mp_patch_E_mo_gyration: null array");
+    const int64_t N = s->n;
+    KernelTimer t("mo_gyration");
+    if (field_dtype == MP_F32 && s->dtype == MP_F32) {
+        mo_gyration_kernel<float, float><<<cdiv(N, 256), 256, 0, stream()>>>(
+            (float *)Ea, (float *)Eb, (float *)s->Pa_from_b, (float *)s->Ja_from_b,
+            (float *)s->Pb_from_a, (float *)s->Jb_from_a, (float *)s->E_old_a, (float *)s->E_old_b,
+            s->idx, (const float *)s->fill, s->mpos, (const float *)iea, (const float *)ieb,
+            (const float *)m_TM, (const float *)m_RE, (const float *)s->poles, s->npoles, N,
+            (float)Q_TM, (float)Q_RE, 2.0f / (float)dt);
+    } else if (field_dtype == MP_F64 && s->dtype == MP_F64) {
+        mo_gyration_kernel<double, double><<<cdiv(N, 256), 256, 0, stream()>>>(
+            (double *)Ea, (double *)Eb, (double *)s->Pa_from_b, (double *)s->Ja_from_b,
+            (double *)s->Pb_from_a, (double *)s->Jb_from_a, (double *)s->E_old_a, (double *)s->E_old_b,
+            s->idx, (const double *)s->fill, s->mpos, (const double *)iea, (const double *)ieb,
+            (const double *)m_TM, (const double *)m_RE, (const double *)s->poles, s->npoles, N, Q_TM,
+            Q_RE, 2.0 / dt);
+    } else if (field_dtype == MP_F32 && s->dtype == MP_F64) {
+        mo_gyration_kernel<float, double><<<cdiv(N, 256), 256, 0, stream()>>>(
+            (float *)Ea, (float *)Eb, (double *)s->Pa_from_b, (double *)s->Ja_from_b,
+            (double *)s->Pb_from_a, (double *)s->Jb_from_a, (double *)s->E_old_a, (double *)s->E_old_b,
+            s->idx, (const double *)s->fill, s->mpos, (const float *)iea, (const float *)ieb,
+            (const double *)m_TM, (const double *)m_RE, (const double *)s->poles, s->npoles, N, Q_TM,
+            Q_RE, 2.0 / dt);
+    } else {
+        return fail_arg("mp_patch_E_mo_gyration: unsupported dtypes");
+    }
+    MP_LAUNCH_CHECK("mo_gyration_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
+                              const mp_ade *az, int32_t has_ade_work, double pabs_factor, double dt) {
+    if (!f || !g) return fail_arg("mp_pabs_accumulate: null argument");
+    if (f->n <= 0) return MP_OK;  // Kernels.jl:629
+    if (has_ade_work) {
+        if (!ax || !ay || !az || ax->n <= 0) has_ade_work = 0;  // Kernels.jl:632-634
+        else if (ax->npoles != ay->npoles || ax->npoles != az->npoles)
+            return fail_arg("mp_pabs_accumulate: ADE lists disagree on npoles");
+    }
+    if (has_ade_work && (!f->pos_x || !f->pos_y || !f->pos_z))
+        return fail_arg("mp_pabs_accumulate: ade_work needs pos_x/pos_y/pos_z");
+    const int32_t fd = g->field_dtype;
+    if (!has_ade_work) {
+        if (fd == MP_F32) return launch_pabs<float, float, float>(f, g, ax, ay, az, 0, pabs_factor, dt);
+        if (fd == MP_F64) return launch_pabs<double, double, double>(f, g, ax, ay, az, 0, pabs_factor, dt);
+        return fail_arg("mp_pabs_accumulate: unsupported field dtype");
+    }
+    const int32_t ct = ax->dtype, sd = ax->state_dtype;
+    if (fd == MP_F32 && ct == MP_F32) {
+        if (sd == MP_F32) return launch_pabs<float, float, float>(f, g, ax, ay, az, 1, pabs_factor, dt);
+        if (sd == MP_BF16) return launch_pabs<float, float, bf16>(f, g, ax, ay, az, 1, pabs_factor, dt);
+        if (sd == MP_F16) return launch_pabs<float, float, f16s>(f, g, ax, ay, az, 1, pabs_factor, dt);
+    } else if (fd == MP_F64 && ct == MP_F64 && sd == MP_F64) {
+        return launch_pabs<double, double, double>(f, g, ax, ay, az, 1, pabs_factor, dt);
+    } else if (fd == MP_F32 && ct == MP_F64 && sd == MP_F64) {
+        return launch_pabs<float, double, double>(f, g, ax, ay, az, 1, pabs_factor, dt);
+    }
+    return fail_arg("mp_pabs_accumulate: unsupported dtypes");
+}

@@ -1,0 +1,330 @@
+// Monitor kernels: running FP64 DFT + net-flux integral on monitor planes, mode-weighted
+// plane traces, plane Poynting flux, film reductions and the non-finite scan.
+//
+// Reference: probe_phase_factors_kernel! / probe_dft_accumulate_kernel! /
+// probe_trace_plane_kernel! (src/fdtd/Kernels.jl:420-478) with _record_probe_device!
+// (src/sim/ProbeReadout.jl:274-301); record!(::FluxMonitor) (src/sim/Monitors.jl:263-341);
+// FilmAverage / SwitchedFraction (Monitors.jl:487-578); NaNGuard (Monitors.jl:580-606).
+// Reductions are two-pass (fixed per-block partials, then one ordered final sum) so
+// they are deterministic run to run.
+#include <mutex>
+
+#include "common.h"
+
+namespace mp {
+
+struct PlaneDesc {
+    int32_t axis;
+    int64_t index, n1, n2;
+    double *ea, *eb, *ha, *hb, *energy;
+};
+struct PlaneSet {
+    PlaneDesc p[MP_MAX_PLANES];
+    int32_t count;
+};
+
+__device__ __forceinline__ int64_t plane_lin(int axis, int64_t idx, int64_t a, int64_t b, int64_t nx, int64_t ny) {
+    if (axis == 0) return idx + nx * (a + ny * b);   // (j, k)
+    if (axis == 1) return b + nx * (idx + ny * a);   // (k, i)
+    return a + nx * (b + ny * idx);                  // (i, j)
+}
+
+__global__ void phase_factors_kernel(double *phase, const double *omega, int nb, double t, double dt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    const double ph = omega[q] * t;
+    phase[2 * q] = cos(ph) * dt;
+    phase[2 * q + 1] = -sin(ph) * dt;
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void dft_accumulate_kernel(
+    PlaneSet ps, const S *Ex, const S *Ey, const S *Ez, const S *Hx, const S *Hy, const S *Hz,
+    const double *__restrict__ phase, int nb, double dt, int64_t nx, int64_t ny) {
+    const PlaneDesc &pl = ps.p[blockIdx.y];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = pl.n1 * pl.n2;
+    if (c >= np) return;
+    const int64_t a = c % pl.n1, b = c / pl.n1;
+    const int64_t o = plane_lin(pl.axis, pl.index, a, b, nx, ny);
+    const S *Ea, *Eb, *Ha, *Hb;
+    if (pl.axis == 0) { Ea = Ey; Eb = Ez; Ha = Hy; Hb = Hz; }
+    else if (pl.axis == 1) { Ea = Ez; Eb = Ex; Ha = Hz; Hb = Hx; }
+    else { Ea = Ex; Eb = Ey; Ha = Hx; Hb = Hy; }
+    const double ea = (double)Ea[o], eb = (double)Eb[o], ha = (double)Ha[o], hb = (double)Hb[o];
+    pl.energy[c] += (ea * hb - eb * ha) * dt;
+    for (int q = 0; q < nb; ++q) {
+        const double pr = phase[2 * q], pi = phase[2 * q + 1];
+        const int64_t z = 2 * (c + np * q);
+        double2 *ea2 = (double2 *)(pl.ea + z), *eb2 = (double2 *)(pl.eb + z);
+        double2 *ha2 = (double2 *)(pl.ha + z), *hb2 = (double2 *)(pl.hb + z);
+        double2 v;
+        v = *ea2; v.x += ea * pr; v.y += ea * pi; *ea2 = v;
+        v = *eb2; v.x += eb * pr; v.y += eb * pi; *eb2 = v;
+        v = *ha2; v.x += ha * pr; v.y += ha * pi; *ha2 = v;
+        v = *hb2; v.x += hb * pr; v.y += hb * pi; *hb2 = v;
+    }
+}
+
+// ---- deterministic reductions -------------------------------------------------------------
+constexpr int RED_BLOCKS = 256;
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAXV = 10;
+
+static double *g_partials = nullptr;  // RED_BLOCKS * RED_MAXV doubles
+static std::mutex g_red_mu;
+
+static int partials(double **out) {
+    std::lock_guard<std::mutex> lk(g_red_mu);
+    if (!g_partials) MP_HIP(hipMalloc((void **)&g_partials, sizeof(double) * RED_BLOCKS * RED_MAXV));
+    *out = g_partials;
+    return MP_OK;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *dst /* NV */) {
+    __shared__ double sh[RED_THREADS];
+    for (int q = 0; q < NV; ++q) {
+        sh[threadIdx.x] = v[q];
+        __syncthreads();
+        for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) dst[q] = sh[0];
+        __syncthreads();
+    }
+}
+
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void trace_partial_kernel(
+    const S *Ea, const S *Eb, int axis, int64_t ipre, int64_t ipost, const double *mode_w,
+    const double *area, int64_t n1, int64_t n2, int64_t nx, int64_t ny, double *part) {
+    double v[4] = {0, 0, 0, 0};
+    const int64_t np = n1 * n2;
+    for (int64_t c = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; c < np; c += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const int64_t a = c % n1, b = c / n1;
+        const double w = mode_w[c] * area[c];
+        const int64_t o1 = plane_lin(axis, ipre, a, b, nx, ny), o2 = plane_lin(axis, ipost, a, b, nx, ny);
+        v[0] += (double)Ea[o1] * w;
+        v[1] += (double)Eb[o1] * w;
+        v[2] += (double)Ea[o2] * w;
+        v[3] += (double)Eb[o2] * w;
+    }
+    block_sum_store<4>(v, part + 4 * blockIdx.x);
+}
+
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void flux_partial_kernel(
+    const S *Ea, const S *Eb, const S *Ha, const S *Hb, int axis, int64_t idx, const double *area,
+    int64_t n1, int64_t n2, int64_t nx, int64_t ny, double *part) {
+    double v[1] = {0};
+    const int64_t np = n1 * n2;
+    for (int64_t c = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; c < np; c += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const int64_t a = c % n1, b = c / n1;
+        const int64_t o = plane_lin(axis, idx, a, b, nx, ny);
+        v[0] += ((double)Ea[o] * (double)Hb[o] - (double)Eb[o] * (double)Ha[o]) * area[c];
+    }
+    block_sum_store<1>(v, part + blockIdx.x);
+}
+
+template <typename L>
+__global__ __launch_bounds__(RED_THREADS) void film_partial_kernel(
+    const L *mTx, const L *mTy, const L *mTz, const L *mRx, const L *mRy, const L *mRz,
+    const L *Te, const L *Tl, int64_t N, double *part) {
+    double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -INFINITY};
+    for (int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; n < N; n += (int64_t)RED_BLOCKS * RED_THREADS) {
+        v[0] += (double)mTx[n]; v[1] += (double)mTy[n]; v[2] += (double)mTz[n];
+        v[3] += (double)mRx[n]; v[4] += (double)mRy[n]; v[5] += (double)mRz[n];
+        const double te = (double)Te[n];
+        v[6] += te; v[7] += (double)Tl[n];
+        v[8] += (double)mTx[n] < 0.0 ? 1.0 : 0.0;
+        v[9] = te > v[9] ? te : v[9];
+    }
+    double s[9];
+    for (int q = 0; q < 9; ++q) s[q] = v[q];
+    block_sum_store<9>(s, part + RED_MAXV * blockIdx.x);
+    __shared__ double mx[RED_THREADS];
+    mx[threadIdx.x] = v[9];
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) mx[threadIdx.x] = mx[threadIdx.x + st] > mx[threadIdx.x] ? mx[threadIdx.x + st] : mx[threadIdx.x];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[RED_MAXV * blockIdx.x + 9] = mx[0];
+}
+
+// Final ordered sum of the partials: out[q] = scale * sum_b part[stride*b + q] (q < nsum),
+// out[q] = max_b part[...] for nsum <= q < nv.
+__global__ void finalize_kernel(const double *part, int stride, int nsum, int nv, double scale, double *out) {
+    const int q = threadIdx.x;
+    if (q >= nv) return;
+    if (q < nsum) {
+        double s = 0.0;
+        for (int b = 0; b < RED_BLOCKS; ++b) s += part[stride * b + q];
+        out[q] = s * scale;
+    } else {
+        double m = -INFINITY;
+        for (int b = 0; b < RED_BLOCKS; ++b) m = part[stride * b + q] > m ? part[stride * b + q] : m;
+        out[q] = m;
+    }
+}
+
+template <typename T>
+__global__ void nonfinite_kernel(const T *x, int64_t n, int32_t *flag) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !__builtin_isfinite((double)x[i]);
+    if (bad) *flag = 1;
+}
+
+static void plane_fields(const mp_yee *g, int axis, const void **Ea, const void **Eb, const void **Ha,
+                         const void **Hb, int64_t *n1, int64_t *n2) {
+    if (axis == 0) { *Ea = g->ey; *Eb = g->ez; *Ha = g->hy; *Hb = g->hz; *n1 = g->ny; *n2 = g->nz; }
+    else if (axis == 1) { *Ea = g->ez; *Eb = g->ex; *Ha = g->hz; *Hb = g->hx; *n1 = g->nz; *n2 = g->nx; }
+    else { *Ea = g->ex; *Eb = g->ey; *Ha = g->hx; *Hb = g->hy; *n1 = g->nx; *n2 = g->ny; }
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+MP_API int mp_probe_dft_accumulate(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
+                                   const double *omega_bins, double *phase, int32_t nb, double t,
+                                   double dt) {
+    if (!g || (!planes && nplanes > 0)) return fail_arg("mp_probe_dft_accumulate: null argument");
+    if (nplanes <= 0 || nb <= 0) return MP_OK;  // Kernels.jl:677-679
+    if (nplanes > MP_MAX_PLANES) return fail_arg("mp_probe_dft_accumulate: at most %d planes", MP_MAX_PLANES);
+    if (!omega_bins || !phase) return fail_arg("mp_probe_dft_accumulate: null bins/phase buffer");
+    PlaneSet ps{};
+    ps.count = nplanes;
+    int64_t maxnp = 0;
+    for (int p = 0; p < nplanes; ++p) {
+        const mp_plane &src = planes[p];
+        if (src.axis < 0 || src.axis > 2) return fail_arg("mp_probe_dft_accumulate: bad plane axis");
+        const int64_t nax = src.axis == 0 ? g->nx : (src.axis == 1 ? g->ny : g->nz);
+        if (src.index < 0 || src.index >= nax) return fail_arg("mp_probe_dft_accumulate: plane index out of range");
+        if (!src.dft_ea || !src.dft_eb || !src.dft_ha || !src.dft_hb || !src.energy)
+            return fail_arg("mp_probe_dft_accumulate: null plane buffer");
+        PlaneDesc &d = ps.p[p];
+        d.axis = src.axis;
+        d.index = src.index;
+        d.n1 = src.axis == 0 ? g->ny : (src.axis == 1 ? g->nz : g->nx);
+        d.n2 = src.axis == 0 ? g->nz : (src.axis == 1 ? g->nx : g->ny);
+        d.ea = src.dft_ea; d.eb = src.dft_eb; d.ha = src.dft_ha; d.hb = src.dft_hb; d.energy = src.energy;
+        maxnp = std::max(maxnp, d.n1 * d.n2);
+    }
+    {
+        KernelTimer t0("dft_phase");
+        phase_factors_kernel<<<cdiv(nb, 64), 64, 0, stream()>>>(phase, omega_bins, nb, t, dt);
+        MP_LAUNCH_CHECK("phase_factors_kernel");
+    }
+    dim3 grid(cdiv(maxnp, 256), nplanes);
+    KernelTimer t1("dft_accumulate");
+    if (g->field_dtype == MP_F32) {
+        dft_accumulate_kernel<float><<<grid, 256, 0, stream()>>>(ps, (const float *)g->ex, (const float *)g->ey,
+            (const float *)g->ez, (const float *)g->hx, (const float *)g->hy, (const float *)g->hz, phase, nb,
+            dt, g->nx, g->ny);
+    } else if (g->field_dtype == MP_F64) {
+        dft_accumulate_kernel<double><<<grid, 256, 0, stream()>>>(ps, (const double *)g->ex, (const double *)g->ey,
+            (const double *)g->ez, (const double *)g->hx, (const double *)g->hy, (const double *)g->hz, phase, nb,
+            dt, g->nx, g->ny);
+    } else {
+        return fail_arg("mp_probe_dft_accumulate: unsupported field dtype");
+    }
+    MP_LAUNCH_CHECK("dft_accumulate_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_probe_trace_plane(const mp_yee *g, int32_t axis, int64_t i_pre, int64_t i_post,
+                                const double *mode_w, const double *area, double *out) {
+    if (!g || !mode_w || !area || !out) return fail_arg("mp_probe_trace_plane: null argument");
+    if (axis < 0 || axis > 2) return fail_arg("mp_probe_trace_plane: bad axis");
+    const int64_t nax = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    if (i_pre < 0 || i_pre >= nax || i_post < 0 || i_post >= nax) return fail_arg("mp_probe_trace_plane: plane index out of range");
+    const void *Ea, *Eb, *Ha, *Hb;
+    int64_t n1, n2;
+    plane_fields(g, axis, &Ea, &Eb, &Ha, &Hb, &n1, &n2);
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("trace_plane");
+    if (g->field_dtype == MP_F32) {
+        trace_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)Ea, (const float *)Eb,
+            axis, i_pre, i_post, mode_w, area, n1, n2, g->nx, g->ny, part);
+    } else if (g->field_dtype == MP_F64) {
+        trace_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)Ea, (const double *)Eb,
+            axis, i_pre, i_post, mode_w, area, n1, n2, g->nx, g->ny, part);
+    } else {
+        return fail_arg("mp_probe_trace_plane: unsupported field dtype");
+    }
+    MP_LAUNCH_CHECK("trace_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, 4, 4, 4, 1.0, out);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const double *area, double *out,
+                         int64_t slot) {
+    if (!g || !area || !out) return fail_arg("mp_flux_plane: null argument");
+    if (axis < 0 || axis > 2) return fail_arg("3-D flux axis must be x, y or z");
+    const int64_t nax = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    if (index < 0 || index >= nax || slot < 0) return fail_arg("mp_flux_plane: index out of range");
+    const void *Ea, *Eb, *Ha, *Hb;
+    int64_t n1, n2;
+    plane_fields(g, axis, &Ea, &Eb, &Ha, &Hb, &n1, &n2);
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("flux_plane");
+    if (g->field_dtype == MP_F32) {
+        flux_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)Ea, (const float *)Eb,
+            (const float *)Ha, (const float *)Hb, axis, index, area, n1, n2, g->nx, g->ny, part);
+    } else if (g->field_dtype == MP_F64) {
+        flux_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)Ea, (const double *)Eb,
+            (const double *)Ha, (const double *)Hb, axis, index, area, n1, n2, g->nx, g->ny, part);
+    } else {
+        return fail_arg("mp_flux_plane: unsupported field dtype");
+    }
+    MP_LAUNCH_CHECK("flux_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, 1, 1, 1, 1.0, out + slot);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_film_reduce(const mp_film *f, double *out) {
+    if (!f || !out) return fail_arg("mp_film_reduce: null argument");
+    if (f->n <= 0) return mp_fill(out, MP_F64, 10, 0.0);
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("film_reduce");
+    if (f->dtype == MP_F32) {
+        film_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)f->m_TM_x,
+            (const float *)f->m_TM_y, (const float *)f->m_TM_z, (const float *)f->m_RE_x, (const float *)f->m_RE_y,
+            (const float *)f->m_RE_z, (const float *)f->Te, (const float *)f->Tl, f->n, part);
+    } else if (f->dtype == MP_F64) {
+        film_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)f->m_TM_x,
+            (const double *)f->m_TM_y, (const double *)f->m_TM_z, (const double *)f->m_RE_x, (const double *)f->m_RE_y,
+            (const double *)f->m_RE_z, (const double *)f->Te, (const double *)f->Tl, f->n, part);
+    } else {
+        return fail_arg("mp_film_reduce: unsupported dtype");
+    }
+    MP_LAUNCH_CHECK("film_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, RED_MAXV, 9, 10, 1.0 / (double)f->n, out);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_any_nonfinite(const void *x, int32_t dtype, int64_t n, int32_t *flag) {
+    if (!flag) return fail_arg("mp_any_nonfinite: null flag");
+    if (n <= 0) return MP_OK;
+    if (!x) return fail_arg("mp_any_nonfinite: null array");
+    const unsigned blocks = (unsigned)std::min<int64_t>(cdiv(n, 256), 4096);
+    KernelTimer t("nonfinite");
+    if (dtype == MP_F32) nonfinite_kernel<float><<<blocks, 256, 0, stream()>>>((const float *)x, n, flag);
+    else if (dtype == MP_F64) nonfinite_kernel<double><<<blocks, 256, 0, stream()>>>((const double *)x, n, flag);
+    else return fail_arg("mp_any_nonfinite: unsupported dtype");
+    MP_LAUNCH_CHECK("nonfinite_kernel");
+    return MP_OK;
+}

@@ -1,0 +1,167 @@
+// Source injection kernels.
+//
+// Reference: inject_soft_3d_kernel! (src/fdtd/Kernels.jl:154-174), inject_mode_x_3d_kernel!
+// (:176-192) and the host loop inject!(::FieldState, ::Grid3D, ::PlaneSource)
+// (src/fdtd/Source.jl:188-236), which the reference runs on the CPU only.
+#include "common.h"
+
+namespace mp {
+
+template <typename S, typename C>
+__global__ void inject_soft_kernel(S *E, S *D, const S *ie, S *H, int64_t o, C val) {
+    if (E) {
+        E[o] = (S)((C)E[o] + val);
+        D[o] = (S)((C)D[o] + val / (C)ie[o]);
+    } else {
+        H[o] = (S)((C)H[o] + val);
+    }
+}
+
+template <typename S, typename C>
+__global__ void inject_mode_x_kernel(S *E, S *D, const S *ie, S *H, const C *profile, int64_t i,
+                                     int64_t nx, int64_t ny, int64_t nz, C value, C hsign_hcorr) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+    if (j >= ny || k >= nz) return;
+    const int64_t o = i + nx * (j + ny * k);
+    const C amp = value * profile[j + ny * k];
+    E[o] = (S)((C)E[o] + amp);
+    D[o] = (S)((C)D[o] + amp / (C)ie[o]);
+    // Ey source: Hz += hcorr*amp; Ez source: Hy -= hcorr*amp (Kernels.jl:186,190)
+    H[o] = (S)((C)H[o] + hsign_hcorr * amp);
+}
+
+// Plane injection: E[c] = S(double(E[c]) + dv), D[c] = S(double(D[c]) + dv / double(ie[c]))
+// over every cell of plane `idx` of `axis` (Source.jl:216-236 promotes to Float64).
+template <typename S>
+__global__ void inject_plane_kernel(S *F, S *D, const S *ie, int axis, int64_t idx, int64_t nx,
+                                    int64_t ny, int64_t nz, double dv) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t b = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
+    int64_t o;
+    if (axis == 0) {        // plane (j, k)
+        if (a >= ny || b >= nz) return;
+        o = idx + nx * (a + ny * b);
+    } else if (axis == 1) {  // plane (i, k)
+        if (a >= nx || b >= nz) return;
+        o = a + nx * (idx + ny * b);
+    } else {                 // plane (i, j)
+        if (a >= nx || b >= ny) return;
+        o = a + nx * (b + ny * idx);
+    }
+    F[o] = (S)((double)F[o] + dv);
+    if (D) D[o] = (S)((double)D[o] + dv / (double)ie[o]);
+}
+
+struct CompPtrs {
+    void *E, *D, *H;
+    const void *ie;
+};
+
+static CompPtrs comp_ptrs(const mp_yee *g, int code) {
+    switch (code) {
+    case 1: return {g->ex, g->dx, nullptr, g->inv_eps_x};
+    case 2: return {g->ey, g->dy, nullptr, g->inv_eps_y};
+    case 3: return {g->ez, g->dz, nullptr, g->inv_eps_z};
+    case 4: return {nullptr, nullptr, g->hx, nullptr};
+    case 5: return {nullptr, nullptr, g->hy, nullptr};
+    case 6: return {nullptr, nullptr, g->hz, nullptr};
+    default: return {nullptr, nullptr, nullptr, nullptr};
+    }
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+MP_API int mp_inject_soft_3d(const mp_yee *g, int32_t component, int64_t i, int64_t j, int64_t k,
+                             double value) {
+    if (!g) return fail_arg("mp_inject_soft_3d: null grid");
+    if (component < 1 || component > 6) return fail_arg("mp_inject_soft_3d: bad component %d", component);
+    if (i < 0 || j < 0 || k < 0 || i >= g->nx || j >= g->ny || k >= g->nz)
+        return fail_arg("mp_inject_soft_3d: index (%lld,%lld,%lld) out of range", (long long)i,
+                        (long long)j, (long long)k);
+    const int64_t o = i + g->nx * (j + g->ny * k);
+    CompPtrs p = comp_ptrs(g, component);
+    KernelTimer t("inject_soft");
+    if (g->field_dtype == MP_F32 && g->compute_dtype == MP_F32) {
+        inject_soft_kernel<float, float><<<1, 1, 0, stream()>>>((float *)p.E, (float *)p.D,
+            (const float *)p.ie, (float *)p.H, o, (float)value);
+    } else if (g->field_dtype == MP_F64 && g->compute_dtype == MP_F64) {
+        inject_soft_kernel<double, double><<<1, 1, 0, stream()>>>((double *)p.E, (double *)p.D,
+            (const double *)p.ie, (double *)p.H, o, value);
+    } else if (g->field_dtype == MP_F32 && g->compute_dtype == MP_F64) {
+        inject_soft_kernel<float, double><<<1, 1, 0, stream()>>>((float *)p.E, (float *)p.D,
+            (const float *)p.ie, (float *)p.H, o, value);
+    } else {
+        return fail_arg("mp_inject_soft_3d: unsupported dtypes");
+    }
+    MP_LAUNCH_CHECK("inject_soft_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_inject_mode_x_3d(const mp_yee *g, int32_t component, int64_t i, const void *profile,
+                               double value, double hcorr) {
+    if (!g || !profile) return fail_arg("mp_inject_mode_x_3d: null argument");
+    if (component != MP_EY && component != MP_EZ)
+        return fail_arg("mp_inject_mode_x_3d: ModeSource electric component must be Ey or Ez");
+    if (i < 0 || i >= g->nx) return fail_arg("mp_inject_mode_x_3d: plane index out of range");
+    void *E = component == MP_EY ? g->ey : g->ez;
+    void *D = component == MP_EY ? g->dy : g->dz;
+    const void *ie = component == MP_EY ? g->inv_eps_y : g->inv_eps_z;
+    void *H = component == MP_EY ? g->hz : g->hy;
+    dim3 block(64, 4);
+    dim3 grid(cdiv(g->ny, 64), cdiv(g->nz, 4));
+    KernelTimer t("inject_mode_x");
+    if (g->field_dtype == MP_F32 && g->compute_dtype == MP_F32) {
+        const float hc = component == MP_EY ? (float)hcorr : -(float)hcorr;
+        inject_mode_x_kernel<float, float><<<grid, block, 0, stream()>>>((float *)E, (float *)D,
+            (const float *)ie, (float *)H, (const float *)profile, i, g->nx, g->ny, g->nz,
+            (float)value, hc);
+    } else if (g->field_dtype == MP_F64 && g->compute_dtype == MP_F64) {
+        const double hc = component == MP_EY ? hcorr : -hcorr;
+        inject_mode_x_kernel<double, double><<<grid, block, 0, stream()>>>((double *)E, (double *)D,
+            (const double *)ie, (double *)H, (const double *)profile, i, g->nx, g->ny, g->nz, value, hc);
+    } else if (g->field_dtype == MP_F32 && g->compute_dtype == MP_F64) {
+        const double hc = component == MP_EY ? hcorr : -hcorr;
+        inject_mode_x_kernel<float, double><<<grid, block, 0, stream()>>>((float *)E, (float *)D,
+            (const float *)ie, (float *)H, (const double *)profile, i, g->nx, g->ny, g->nz, value, hc);
+    } else {
+        return fail_arg("mp_inject_mode_x_3d: unsupported dtypes");
+    }
+    MP_LAUNCH_CHECK("inject_mode_x_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_inject_plane_3d(const mp_yee *g, int32_t component, int32_t axis, int64_t i0,
+                              double w0, int64_t i1, double w1, double value) {
+    if (!g) return fail_arg("mp_inject_plane_3d: null grid");
+    if (component < 1 || component > 6) return fail_arg("mp_inject_plane_3d: bad component %d", component);
+    if (axis < 0 || axis > 2) return fail_arg("mp_inject_plane_3d: 3-D PlaneSource axis must be x, y or z");
+    const int64_t n_ax = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    if (i0 < 0 || i0 >= n_ax || i1 >= n_ax) return fail_arg("mp_inject_plane_3d: plane index out of range");
+    CompPtrs p = comp_ptrs(g, component);
+    void *F = p.E ? p.E : p.H;
+    const int64_t n1 = axis == 0 ? g->ny : g->nx;
+    const int64_t n2 = axis == 2 ? g->ny : g->nz;
+    dim3 block(64, 4);
+    dim3 grid(cdiv(n1, 64), cdiv(n2, 4));
+    const int64_t idxs[2] = {i0, i1};
+    const double ws[2] = {w0, w1};
+    KernelTimer t("inject_plane");
+    for (int s = 0; s < 2; ++s) {
+        if (idxs[s] < 0) continue;
+        const double dv = ws[s] * value;
+        if (g->field_dtype == MP_F32) {
+            inject_plane_kernel<float><<<grid, block, 0, stream()>>>((float *)F, (float *)p.D,
+                (const float *)p.ie, axis, idxs[s], g->nx, g->ny, g->nz, dv);
+        } else if (g->field_dtype == MP_F64) {
+            inject_plane_kernel<double><<<grid, block, 0, stream()>>>((double *)F, (double *)p.D,
+                (const double *)p.ie, axis, idxs[s], g->nx, g->ny, g->nz, dv);
+        } else {
+            return fail_arg("mp_inject_plane_3d: unsupported field dtype");
+        }
+        MP_LAUNCH_CHECK("inject_plane_kernel");
+    }
+    return MP_OK;
+}

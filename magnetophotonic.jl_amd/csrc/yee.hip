@@ -1,0 +1,316 @@
+// Yee H / E leapfrog with CFS-CPML convolution memory, FP32 (or FP64) on gfx950.
+//
+// Reference: src/fdtd/Kernels.jl:9-152 (yee_H_3d_kernel!, yee_E_3d_kernel!,
+// _cpml_term_{x,y,z}!, _pml_slab) and the launchers :522-565.
+//
+// Layout: volumes column-major, x fastest.  A workgroup is a 64 x 4 tile of (i, j)
+// columns (one wave64 per x-row, so every global access of a wave is 64 consecutive
+// elements = 256 B for FP32); each thread marches a chunk of z planes and keeps the
+// previous plane's values in registers, so the z-neighbour never comes from memory
+// twice.  x/y neighbours are re-read from the same or the adjacent rows and are served
+// by the L1/L2 caches.  CPML psi slabs are touched only by threads inside a slab.
+//
+// Arithmetic order is the reference's: derivative = (E[n] - E[c]) * inv_d, psi update
+// = fma(b, psi, a*deriv) (Julia muladd), curl terms and the D/H updates exactly as
+// written there, compiled with -ffp-contract=off.
+#include "common.h"
+
+namespace mp {
+
+constexpr int TX = 64;
+constexpr int TY = 4;
+
+template <typename C, typename P> struct PsiIO {
+    static __device__ __forceinline__ C load(const P *p, int64_t o, float) { return (C)p[o]; }
+    static __device__ __forceinline__ void store(P *p, int64_t o, C v, float) { p[o] = (P)v; }
+};
+template <> struct PsiIO<float, bf16> {
+    static __device__ __forceinline__ float load(const bf16 *p, int64_t o, float inv_sc) { return ld<float>(p[o]) * inv_sc; }
+    static __device__ __forceinline__ void store(bf16 *p, int64_t o, float v, float sc) { p[o] = st<bf16>(v * sc); }
+};
+template <> struct PsiIO<float, f16s> {
+    static __device__ __forceinline__ float load(const f16s *p, int64_t o, float inv_sc) { return ld<float>(p[o]) * inv_sc; }
+    static __device__ __forceinline__ void store(f16s *p, int64_t o, float v, float sc) { p[o] = st<f16s>(v * sc); }
+};
+
+// _pml_slab (Kernels.jl:9-14), 0-based: region 0 none, 1 lo, 2 hi; *js slab coordinate.
+__device__ __forceinline__ int pml_slab(int64_t idx, int64_t N, int64_t w, int64_t *js) {
+    if (w <= 0) { *js = 0; return 0; }
+    if (idx < w) { *js = idx; return 1; }
+    if (idx >= N - w) { *js = idx - (N - w); return 2; }
+    *js = 0;
+    return 0;
+}
+
+// _cpml_term_*! (Kernels.jl:16-56)
+template <typename C, typename P>
+__device__ __forceinline__ C cpml_term(P *lo, P *hi, int region, int64_t off, C b, C a, C deriv,
+                                       float sc, float isc) {
+    if (region == 1) {
+        C p = fmaT<C>(b, PsiIO<C, P>::load(lo, off, isc), a * deriv);
+        PsiIO<C, P>::store(lo, off, p, sc);
+        return p;
+    } else if (region == 2) {
+        C p = fmaT<C>(b, PsiIO<C, P>::load(hi, off, isc), a * deriv);
+        PsiIO<C, P>::store(hi, off, p, sc);
+        return p;
+    }
+    return (C)0;
+}
+
+struct CpmlArgs {
+    const void *ik[3], *a[3], *b[3];
+    void *psi[12];
+    int64_t w[3];
+    float sc, isc;
+};
+
+template <typename S, typename C, typename P>
+__global__ __launch_bounds__(TX *TY) void yee_H_kernel(
+    const S *__restrict__ Ex, const S *__restrict__ Ey, const S *__restrict__ Ez,
+    S *__restrict__ Hx, S *__restrict__ Hy, S *__restrict__ Hz,
+    const C *__restrict__ idcx, const C *__restrict__ idcy, const C *__restrict__ idcz,
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, C dt_mu0) {
+    const int64_t i = (int64_t)blockIdx.x * TX + threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.y * TY + threadIdx.y;
+    if (i >= nx - 1 || j >= ny - 1) return;
+    const int64_t k0 = (int64_t)blockIdx.z * kchunk;
+    const int64_t k1 = min(k0 + kchunk, nz - 1);
+    if (k0 >= k1) return;
+    const int64_t sy = nx, sz = nx * ny;
+    const C *xk = (const C *)cp.ik[0], *xa = (const C *)cp.a[0], *xb = (const C *)cp.b[0];
+    const C *yk = (const C *)cp.ik[1], *ya = (const C *)cp.a[1], *yb = (const C *)cp.b[1];
+    const C *zk = (const C *)cp.ik[2], *za = (const C *)cp.a[2], *zb = (const C *)cp.b[2];
+    P **ps = (P **)cp.psi;
+    const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
+
+    int64_t jsx, jsy;
+    const int rx = pml_slab(i, nx, wx, &jsx);
+    const int ry = pml_slab(j, ny, wy, &jsy);
+    const C idx_ = idcx[i], idy_ = idcy[j];
+    const C xk_ = xk[i], yk_ = yk[j];
+    const C xa_ = xa[i], xb_ = xb[i], ya_ = ya[j], yb_ = yb[j];
+
+    int64_t o = i + sy * j + sz * k0;
+    C ex_c = (C)Ex[o], ey_c = (C)Ey[o];
+    for (int64_t k = k0; k < k1; ++k, o += sz) {
+        const C ex_n = (C)Ex[o + sz], ey_n = (C)Ey[o + sz];
+        const C ez_c = (C)Ez[o];
+        const C idz_ = idcz[k];
+        const C dEz_dy = ((C)Ez[o + sy] - ez_c) * idy_;
+        const C dEy_dz = (ey_n - ey_c) * idz_;
+        const C dEx_dz = (ex_n - ex_c) * idz_;
+        const C dEz_dx = ((C)Ez[o + 1] - ez_c) * idx_;
+        const C dEy_dx = ((C)Ey[o + 1] - ey_c) * idx_;
+        const C dEx_dy = ((C)Ex[o + sy] - ex_c) * idy_;
+
+        int64_t jsz;
+        const int rz = pml_slab(k, nz, wz, &jsz);
+        const int64_t ox = jsx + wx * (j + ny * k);
+        const int64_t oy = i + nx * (jsy + wy * k);
+        const int64_t oz = i + nx * (j + ny * jsz);
+        const C zk_ = zk[k];
+        const C za_ = rz ? za[k] : (C)0, zb_ = rz ? zb[k] : (C)0;
+
+        const C pHxy = cpml_term<C, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
+        const C pHxz = cpml_term<C, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
+        Hx[o] = (S)((C)Hx[o] - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz)));
+
+        const C pHyz = cpml_term<C, P>(ps[4], ps[5], rz, oz, zb_, za_, dEx_dz, cp.sc, cp.isc);
+        const C pHyx = cpml_term<C, P>(ps[6], ps[7], rx, ox, xb_, xa_, dEz_dx, cp.sc, cp.isc);
+        Hy[o] = (S)((C)Hy[o] - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx)));
+
+        const C pHzx = cpml_term<C, P>(ps[8], ps[9], rx, ox, xb_, xa_, dEy_dx, cp.sc, cp.isc);
+        const C pHzy = cpml_term<C, P>(ps[10], ps[11], ry, oy, yb_, ya_, dEx_dy, cp.sc, cp.isc);
+        Hz[o] = (S)((C)Hz[o] - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy)));
+
+        ex_c = ex_n;
+        ey_c = ey_n;
+    }
+}
+
+template <typename S, typename C, typename P>
+__global__ __launch_bounds__(TX *TY) void yee_E_kernel(
+    S *__restrict__ Ex, S *__restrict__ Ey, S *__restrict__ Ez,
+    const S *__restrict__ Hx, const S *__restrict__ Hy, const S *__restrict__ Hz,
+    S *__restrict__ Dx, S *__restrict__ Dy, S *__restrict__ Dz,
+    const S *__restrict__ iex, const S *__restrict__ iey, const S *__restrict__ iez,
+    const C *__restrict__ iddx, const C *__restrict__ iddy, const C *__restrict__ iddz,
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, C dt) {
+    const int64_t i = (int64_t)blockIdx.x * TX + threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.y * TY + threadIdx.y;
+    if (i < 1 || j < 1 || i >= nx || j >= ny) return;
+    const int64_t k0 = max((int64_t)blockIdx.z * kchunk, (int64_t)1);
+    const int64_t k1 = min((int64_t)blockIdx.z * kchunk + kchunk, nz);
+    if (k0 >= k1) return;
+    const int64_t sy = nx, sz = nx * ny;
+    const C *xk = (const C *)cp.ik[0], *xa = (const C *)cp.a[0], *xb = (const C *)cp.b[0];
+    const C *yk = (const C *)cp.ik[1], *ya = (const C *)cp.a[1], *yb = (const C *)cp.b[1];
+    const C *zk = (const C *)cp.ik[2], *za = (const C *)cp.a[2], *zb = (const C *)cp.b[2];
+    P **ps = (P **)cp.psi;
+    const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
+
+    int64_t jsx, jsy;
+    const int rx = pml_slab(i, nx, wx, &jsx);
+    const int ry = pml_slab(j, ny, wy, &jsy);
+    const C idx_ = iddx[i], idy_ = iddy[j];
+    const C xk_ = xk[i], yk_ = yk[j];
+    const C xa_ = xa[i], xb_ = xb[i], ya_ = ya[j], yb_ = yb[j];
+
+    int64_t o = i + sy * j + sz * k0;
+    C hx_p = (C)Hx[o - sz], hy_p = (C)Hy[o - sz];
+    for (int64_t k = k0; k < k1; ++k, o += sz) {
+        const C hx_c = (C)Hx[o], hy_c = (C)Hy[o], hz_c = (C)Hz[o];
+        const C idz_ = iddz[k];
+        const C dHz_dy = (hz_c - (C)Hz[o - sy]) * idy_;
+        const C dHy_dz = (hy_c - hy_p) * idz_;
+        const C dHx_dz = (hx_c - hx_p) * idz_;
+        const C dHz_dx = (hz_c - (C)Hz[o - 1]) * idx_;
+        const C dHy_dx = (hy_c - (C)Hy[o - 1]) * idx_;
+        const C dHx_dy = (hx_c - (C)Hx[o - sy]) * idy_;
+
+        int64_t jsz;
+        const int rz = pml_slab(k, nz, wz, &jsz);
+        const int64_t ox = jsx + wx * (j + ny * k);
+        const int64_t oy = i + nx * (jsy + wy * k);
+        const int64_t oz = i + nx * (j + ny * jsz);
+        const C zk_ = zk[k];
+        const C za_ = rz ? za[k] : (C)0, zb_ = rz ? zb[k] : (C)0;
+
+        const C pDxy = cpml_term<C, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
+        const C pDxz = cpml_term<C, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
+        const C cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
+
+        const C pDyz = cpml_term<C, P>(ps[4], ps[5], rz, oz, zb_, za_, dHx_dz, cp.sc, cp.isc);
+        const C pDyx = cpml_term<C, P>(ps[6], ps[7], rx, ox, xb_, xa_, dHz_dx, cp.sc, cp.isc);
+        const C cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
+
+        const C pDzx = cpml_term<C, P>(ps[8], ps[9], rx, ox, xb_, xa_, dHy_dx, cp.sc, cp.isc);
+        const C pDzy = cpml_term<C, P>(ps[10], ps[11], ry, oy, yb_, ya_, dHx_dy, cp.sc, cp.isc);
+        const C cz_term = (dHy_dx * xk_ + pDzx) - (dHx_dy * yk_ + pDzy);
+
+        const C dx_new = (C)Dx[o] + dt * cx_term;
+        const C dy_new = (C)Dy[o] + dt * cy_term;
+        const C dz_new = (C)Dz[o] + dt * cz_term;
+        Dx[o] = (S)dx_new;
+        Dy[o] = (S)dy_new;
+        Dz[o] = (S)dz_new;
+        Ex[o] = (S)(dx_new * (C)iex[o]);
+        Ey[o] = (S)(dy_new * (C)iey[o]);
+        Ez[o] = (S)(dz_new * (C)iez[o]);
+
+        hx_p = hx_c;
+        hy_p = hy_c;
+    }
+}
+
+// z-chunk length: enough workgroups to fill 256 CUs several times over.
+static int64_t pick_kchunk(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t tiles = (int64_t)cdiv(nx, TX) * (int64_t)cdiv(ny, TY);
+    const int64_t target = 8192;  // workgroups per launch
+    int64_t nchunks = std::max<int64_t>(1, std::min<int64_t>(nz, target / std::max<int64_t>(tiles, 1)));
+    int64_t kc = (nz + nchunks - 1) / nchunks;
+    return std::max<int64_t>(kc, 4);
+}
+
+static CpmlArgs make_cpml_args(const mp_cpml *c, bool h) {
+    CpmlArgs a;
+    for (int d = 0; d < 3; ++d) {
+        a.ik[d] = c->inv_kappa[d];
+        a.a[d] = c->a[d];
+        a.b[d] = c->b[d];
+    }
+    for (int s = 0; s < 12; ++s) a.psi[s] = h ? c->psi_h[s] : c->psi_d[s];
+    a.w[0] = c->npml_x;
+    a.w[1] = c->npml_y;
+    a.w[2] = c->npml_z;
+    const double sc = (c->psi_dtype == MP_BF16 || c->psi_dtype == MP_F16) ? c->psi_scale : 1.0;
+    a.sc = (float)(sc > 0 ? sc : 1.0);
+    a.isc = (float)(1.0 / (sc > 0 ? sc : 1.0));
+    return a;
+}
+
+static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
+    if (!g || !c) return fail_arg("%s: null argument", who);
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail_arg("%s: bad dims", who);
+    if (g->nx * g->ny * g->nz >= (int64_t)1 << 40) return fail_arg("%s: grid too large", who);
+    const int64_t w[3] = {c->npml_x, c->npml_y, c->npml_z};
+    const int64_t n[3] = {g->nx, g->ny, g->nz};
+    for (int d = 0; d < 3; ++d) {
+        if (w[d] < 0 || (w[d] > 0 && n[d] < 2 * w[d] + 1)) return fail_arg("%s: bad CPML width on axis %d", who, d);
+        if (!c->inv_kappa[d] || !c->a[d] || !c->b[d]) return fail_arg("%s: missing CPML coefficients", who);
+    }
+    return MP_OK;
+}
+
+template <typename S, typename C, typename P>
+static int launch_H(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int64_t kc = pick_kchunk(nx, ny, nz);
+    dim3 grid(cdiv(nx, TX), cdiv(ny, TY), cdiv(nz, kc));
+    dim3 block(TX, TY, 1);
+    const C dt_mu0 = (C)dt / (C)mu0;
+    KernelTimer t("yee_H");
+    yee_H_kernel<S, C, P><<<grid, block, 0, stream()>>>(
+        (const S *)g->ex, (const S *)g->ey, (const S *)g->ez, (S *)g->hx, (S *)g->hy, (S *)g->hz,
+        (const C *)g->inv_d_cell_x, (const C *)g->inv_d_cell_y, (const C *)g->inv_d_cell_z,
+        make_cpml_args(c, true), nx, ny, nz, kc, dt_mu0);
+    MP_LAUNCH_CHECK("yee_H_kernel");
+    return MP_OK;
+}
+
+template <typename S, typename C, typename P>
+static int launch_E(const mp_yee *g, const mp_cpml *c, double dt) {
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int64_t kc = pick_kchunk(nx, ny, nz);
+    dim3 grid(cdiv(nx, TX), cdiv(ny, TY), cdiv(nz, kc));
+    dim3 block(TX, TY, 1);
+    KernelTimer t("yee_E");
+    yee_E_kernel<S, C, P><<<grid, block, 0, stream()>>>(
+        (S *)g->ex, (S *)g->ey, (S *)g->ez, (const S *)g->hx, (const S *)g->hy, (const S *)g->hz,
+        (S *)g->dx, (S *)g->dy, (S *)g->dz, (const S *)g->inv_eps_x, (const S *)g->inv_eps_y,
+        (const S *)g->inv_eps_z, (const C *)g->inv_d_dual_x, (const C *)g->inv_d_dual_y,
+        (const C *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz, kc, (C)dt);
+    MP_LAUNCH_CHECK("yee_E_kernel");
+    return MP_OK;
+}
+
+}  // namespace mp
+
+using namespace mp;
+
+MP_API int mp_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0) {
+    int rc = check_yee(g, c, "mp_update_H_3d");
+    if (rc) return rc;
+    if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:528
+    const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
+    if (fd == MP_F32 && cd == MP_F32) {
+        if (pd == MP_F32) return launch_H<float, float, float>(g, c, dt, mu0);
+        if (pd == MP_BF16) return launch_H<float, float, bf16>(g, c, dt, mu0);
+        if (pd == MP_F16) return launch_H<float, float, f16s>(g, c, dt, mu0);
+    } else if (fd == MP_F64 && cd == MP_F64) {
+        if (pd == MP_F64) return launch_H<double, double, double>(g, c, dt, mu0);
+        if (pd == MP_F32) return launch_H<double, double, float>(g, c, dt, mu0);
+    } else if (fd == MP_F32 && cd == MP_F64) {
+        if (pd == MP_F32) return launch_H<float, double, float>(g, c, dt, mu0);
+    }
+    return fail_arg("mp_update_H_3d: unsupported dtypes field=%d compute=%d psi=%d", fd, cd, pd);
+}
+
+MP_API int mp_update_E_3d(const mp_yee *g, mp_cpml *c, double dt) {
+    int rc = check_yee(g, c, "mp_update_E_3d");
+    if (rc) return rc;
+    if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:550
+    const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
+    if (fd == MP_F32 && cd == MP_F32) {
+        if (pd == MP_F32) return launch_E<float, float, float>(g, c, dt);
+        if (pd == MP_BF16) return launch_E<float, float, bf16>(g, c, dt);
+        if (pd == MP_F16) return launch_E<float, float, f16s>(g, c, dt);
+    } else if (fd == MP_F64 && cd == MP_F64) {
+        if (pd == MP_F64) return launch_E<double, double, double>(g, c, dt);
+        if (pd == MP_F32) return launch_E<double, double, float>(g, c, dt);
+    } else if (fd == MP_F32 && cd == MP_F64) {
+        if (pd == MP_F32) return launch_E<float, double, float>(g, c, dt);
+    }
+    return fail_arg("mp_update_E_3d: unsupported dtypes field=%d compute=%d psi=%d", fd, cd, pd);
+}

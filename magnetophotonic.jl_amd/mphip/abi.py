@@ -1,0 +1,111 @@
+"""ctypes mirror of include/mphip.h (struct layouts and the C-ABI function table).
+
+The layouts are shared by anything that calls a library exporting this ABI; the loader
+for libmphip itself lives in `_lib.py`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+MP_F32, MP_F64, MP_BF16, MP_F16 = 0, 1, 2, 3
+MP_MAX_PLANES = 8
+
+vp = C.c_void_p
+i32 = C.c_int32
+i64 = C.c_int64
+f64 = C.c_double
+
+
+class Yee(C.Structure):
+    _fields_ = [("nx", i64), ("ny", i64), ("nz", i64), ("field_dtype", i32), ("compute_dtype", i32),
+                ("ex", vp), ("ey", vp), ("ez", vp), ("hx", vp), ("hy", vp), ("hz", vp),
+                ("dx", vp), ("dy", vp), ("dz", vp),
+                ("inv_eps_x", vp), ("inv_eps_y", vp), ("inv_eps_z", vp),
+                ("inv_d_cell_x", vp), ("inv_d_cell_y", vp), ("inv_d_cell_z", vp),
+                ("inv_d_dual_x", vp), ("inv_d_dual_y", vp), ("inv_d_dual_z", vp)]
+
+
+class Cpml(C.Structure):
+    _fields_ = [("npml_x", i32), ("npml_y", i32), ("npml_z", i32), ("psi_dtype", i32),
+                ("inv_kappa", vp * 3), ("a", vp * 3), ("b", vp * 3),
+                ("psi_h", vp * 12), ("psi_d", vp * 12), ("psi_scale", f64)]
+
+
+class Ade(C.Structure):
+    _fields_ = [("n", i64), ("npoles", i32), ("dtype", i32), ("state_dtype", i32), ("_pad", i32),
+                ("state_scale", f64), ("P", vp), ("J", vp), ("E_old", vp), ("idx", vp),
+                ("fill", vp), ("inv_eps", vp), ("poles", vp)]
+
+
+class Mo(C.Structure):
+    _fields_ = [("n", i64), ("npoles", i32), ("dtype", i32),
+                ("Pa_from_b", vp), ("Ja_from_b", vp), ("Pb_from_a", vp), ("Jb_from_a", vp),
+                ("E_old_a", vp), ("E_old_b", vp), ("idx", vp), ("fill", vp), ("mpos", vp),
+                ("poles", vp)]
+
+
+class Film(C.Structure):
+    _fields_ = [("n", i64), ("dtype", i32), ("_pad", i32), ("cells", vp),
+                ("pos_x", vp), ("pos_y", vp), ("pos_z", vp),
+                ("Te", vp), ("Tl", vp), ("Ts_TM", vp), ("Ts_RE", vp),
+                ("m_TM_x", vp), ("m_TM_y", vp), ("m_TM_z", vp),
+                ("m_RE_x", vp), ("m_RE_y", vp), ("m_RE_z", vp),
+                ("U_abs", vp), ("window_sum", vp), ("peak", vp)]
+
+
+class Lut(C.Structure):
+    _fields_ = [("tm", vp), ("re", vp), ("T_min", f64), ("inv_dT", f64), ("n", i32), ("dtype", i32)]
+
+
+class Plane(C.Structure):
+    _fields_ = [("axis", i32), ("_pad", i32), ("index", i64), ("dft_ea", vp), ("dft_eb", vp),
+                ("dft_ha", vp), ("dft_hb", vp), ("energy", vp)]
+
+
+P = C.POINTER
+SIGNATURES = {
+    "update_H_3d": (C.c_int, [P(Yee), P(Cpml), f64, f64]),
+    "update_E_3d": (C.c_int, [P(Yee), P(Cpml), f64]),
+    "inject_soft_3d": (C.c_int, [P(Yee), i32, i64, i64, i64, f64]),
+    "inject_mode_x_3d": (C.c_int, [P(Yee), i32, i64, vp, f64, f64]),
+    "inject_plane_3d": (C.c_int, [P(Yee), i32, i32, i64, f64, i64, f64, f64]),
+    "patch_E_dispersive": (C.c_int, [vp, P(Ade), i32, f64]),
+    "patch_E_mo_gyration": (C.c_int, [vp, vp, P(Mo), vp, vp, i32, vp, vp, f64, f64, f64]),
+    "pabs_accumulate": (C.c_int, [P(Film), P(Yee), P(Ade), P(Ade), P(Ade), i32, f64, f64]),
+    "multiphysics_step": (C.c_int, [P(Film), vp, P(Lut), f64, i32, i32, i32]),
+    "probe_dft_accumulate": (C.c_int, [P(Yee), P(Plane), i32, vp, vp, i32, f64, f64]),
+    "probe_trace_plane": (C.c_int, [P(Yee), i32, i64, i64, vp, vp, vp]),
+    "flux_plane": (C.c_int, [P(Yee), i32, i64, vp, vp, i64]),
+    "film_reduce": (C.c_int, [P(Film), vp]),
+    "last_error": (C.c_char_p, []),
+}
+
+RUNTIME_SIGNATURES = {
+    "mp_init": (C.c_int, [C.c_int]),
+    "mp_malloc": (C.c_int, [P(vp), C.c_size_t]),
+    "mp_free": (C.c_int, [vp]),
+    "mp_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+    "mp_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+    "mp_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+    "mp_memset": (C.c_int, [vp, C.c_int, C.c_size_t]),
+    "mp_fill": (C.c_int, [vp, i32, i64, f64]),
+    "mp_synchronize": (C.c_int, []),
+    "mp_mem_info": (C.c_int, [P(C.c_size_t), P(C.c_size_t)]),
+    "mp_set_stream": (C.c_int, [vp]),
+    "mp_get_stream": (vp, []),
+    "mp_event_create": (C.c_int, [P(vp)]),
+    "mp_event_destroy": (C.c_int, [vp]),
+    "mp_event_record": (C.c_int, [vp]),
+    "mp_event_elapsed_ms": (C.c_int, [vp, vp, P(C.c_float)]),
+    "mp_profile_enable": (C.c_int, [C.c_int]),
+    "mp_kernel_stats": (C.c_int, [C.c_char_p, P(i64), P(f64)]),
+    "mp_any_nonfinite": (C.c_int, [vp, i32, i64, vp]),
+}
+
+
+def bind(lib, prefix: str, table: dict) -> None:
+    """Attach restype/argtypes for every `prefix + name` entry of `table`."""
+    for name, (res, args) in table.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype = res
+        fn.argtypes = args

@@ -1,0 +1,429 @@
+"""FDTDState on the MI355X: device-resident coupled-step state and the step sequence.
+
+Mirrors src/fdtd/Solver.jl: the data model (:9-53), `step!`/`_step_typed!` (:277-369),
+the multiphysics window (:373-390), `relax_step!` (:548-555), `run!` (:557-563),
+`magnetization_snapshot` / `apply_magnetization!` (:448-483), `configure_probe_mode!`
+(:416-443) and `free_state!` (:521-546).  Every per-step operation is a libmphip call on
+one stream; the host never touches field data inside the loop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional
+
+import numpy as np
+
+from . import _lib, abi, dispersion
+from .sources import (AXIS_CODE, COMPONENT_CODE, ModeSource, PlaneSource, SoftSource, axis_weights,
+                      source_index)
+from .spec import StateSpec
+
+_DT_CODE = {np.dtype(np.float32): abi.MP_F32, np.dtype(np.float64): abi.MP_F64}
+_PSI_CODE = {"f32": abi.MP_F32, "f64": abi.MP_F64, "bf16": abi.MP_BF16, "f16": abi.MP_F16}
+_PSI_SIZE = {"f32": 4, "f64": 8, "bf16": 2, "f16": 2}
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (finalised by `free`)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _lib.call("malloc", C.byref(p), C.c_size_t(max(self.nbytes, 0)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            _lib.call("memcpy_h2d", C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes))
+        return b
+
+    @classmethod
+    def zeros(cls, nbytes: int) -> "DeviceBuffer":
+        b = cls(nbytes)
+        if nbytes:
+            _lib.call("memset", C.c_void_p(b.ptr), 0, C.c_size_t(nbytes))
+        return b
+
+    def upload(self, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {a.nbytes} vs {self.nbytes}")
+        if a.nbytes:
+            _lib.call("memcpy_h2d", C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes))
+
+    def download(self, dtype, count: Optional[int] = None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dtype)
+        if out.nbytes:
+            _lib.call("memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), C.c_size_t(out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _lib.call("free", C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    @property
+    def vp(self):
+        return C.c_void_p(self.ptr) if self.ptr else None
+
+
+FIELD_NAMES = ("ex", "ey", "ez", "hx", "hy", "hz", "dx", "dy", "dz")
+FILM_NAMES = ("Te", "Tl", "Ts_TM", "Ts_RE", "m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z")
+
+
+class FDTDState:
+    """Device state of one coupled 3-D simulation; `step()` advances one EM step."""
+
+    def __init__(self, spec: StateSpec, device: int = 0):
+        _lib.init(device)
+        self.spec = spec
+        self.n = 0
+        self.t = 0.0
+        self.last_mp_n = 0
+        self.multiphysics_enabled = spec.multiphysics_enabled
+        self.dt = spec.dt
+        self._bufs = []
+        nx, ny, nz = spec.shape
+        ncell = nx * ny * nz
+        FT, CT = spec.field_dtype, spec.compute_dtype
+        self._fcode, self._ccode = _DT_CODE[FT], _DT_CODE[CT]
+
+        def dev_zeros(nbytes):
+            b = DeviceBuffer.zeros(nbytes)
+            self._bufs.append(b)
+            return b
+
+        def dev_arr(a):
+            b = DeviceBuffer.from_array(a)
+            self._bufs.append(b)
+            return b
+
+        # Yee volumes and metric arrays
+        self.f = {k: dev_zeros(ncell * FT.itemsize) for k in FIELD_NAMES}
+        self.inv_eps = [dev_arr(a) for a in spec.inv_eps]
+        self.idc = [dev_arr(a) for a in spec.inv_d_cell]
+        self.idd = [dev_arr(a) for a in spec.inv_d_dual]
+        y = abi.Yee()
+        y.nx, y.ny, y.nz = nx, ny, nz
+        y.field_dtype, y.compute_dtype = self._fcode, self._ccode
+        for k in FIELD_NAMES:
+            setattr(y, k, self.f[k].ptr)
+        y.inv_eps_x, y.inv_eps_y, y.inv_eps_z = (b.ptr for b in self.inv_eps)
+        y.inv_d_cell_x, y.inv_d_cell_y, y.inv_d_cell_z = (b.ptr for b in self.idc)
+        y.inv_d_dual_x, y.inv_d_dual_y, y.inv_d_dual_z = (b.ptr for b in self.idd)
+        self.yee = y
+
+        # CPML
+        cs = spec.cpml
+        cp = abi.Cpml()
+        cp.npml_x, cp.npml_y, cp.npml_z = cs.widths
+        cp.psi_dtype = _PSI_CODE[spec.psi_dtype]
+        cp.psi_scale = spec.psi_scale
+        self._cpml_coef = []
+        for d, ax in enumerate(cs.axes):
+            bk, ba, bb = dev_arr(ax.inv_kappa), dev_arr(ax.a), dev_arr(ax.b)
+            cp.inv_kappa[d], cp.a[d], cp.b[d] = bk.ptr, ba.ptr, bb.ptr
+        from .cpml import PSI_AXIS
+        esz = _PSI_SIZE[spec.psi_dtype]
+        self.psi_h, self.psi_d = [], []
+        for s in range(12):
+            nb = cs.slab_cells(PSI_AXIS[s]) * esz
+            bh, bd = dev_zeros(nb), dev_zeros(nb)
+            self.psi_h.append(bh)
+            self.psi_d.append(bd)
+            cp.psi_h[s] = bh.ptr or None
+            cp.psi_d[s] = bd.ptr or None
+        self.cpml = cp
+
+        # source
+        self._mode_profile = None
+        src = spec.source
+        if isinstance(src, ModeSource):
+            if src.axis != "x":
+                raise ValueError("ModeSource injection supports axis='x'")  # Kernels.jl:581
+            if src.component not in ("Ey", "Ez"):
+                raise ValueError("ModeSource electric component must be Ey or Ez")
+            prof = np.asarray(src.profile, dtype=np.float64).reshape(-1, order="F").astype(CT)
+            self._mode_profile = dev_arr(prof)
+            self._mode_i = source_index(spec.grid.x, src.position)
+            eta0 = math.sqrt(spec.params.mu0 / spec.params.eps0)
+            self._mode_hcorr = src.neff / eta0
+
+        # ADE lists
+        self.ade = None
+        self.ade_bufs = None
+        if spec.ade is not None:
+            np_ = len(spec.diag_poles) // 3
+            self.poles_dev = dev_arr(spec.diag_poles)
+            sd = spec.ade_state_dtype
+            ssz = _PSI_SIZE[sd]
+            self.ade = []
+            self.ade_bufs = []
+            for lst in spec.ade:
+                n = len(lst.idx)
+                bufs = dict(P=dev_zeros(n * np_ * ssz), J=dev_zeros(n * np_ * ssz),
+                            E_old=dev_zeros(n * CT.itemsize), idx=dev_arr(lst.idx),
+                            fill=dev_arr(lst.fill), inv=dev_arr(lst.inv))
+                a = abi.Ade()
+                a.n, a.npoles, a.dtype = n, np_, self._ccode
+                a.state_dtype, a.state_scale = _PSI_CODE[sd], spec.ade_state_scale
+                a.P, a.J, a.E_old = bufs["P"].ptr, bufs["J"].ptr, bufs["E_old"].ptr
+                a.idx, a.fill, a.inv_eps = bufs["idx"].ptr, bufs["fill"].ptr, bufs["inv"].ptr
+                a.poles = self.poles_dev.ptr
+                self.ade.append(a)
+                self.ade_bufs.append(bufs)
+
+        # film state
+        self.film = None
+        if spec.has_film:
+            n = spec.n_film
+            self.film_bufs = {k: dev_arr(spec.film_init[k]) for k in FILM_NAMES}
+            self.film_bufs["cells"] = dev_arr(spec.film_cells)
+            for c, pos in zip("xyz", spec.pos):
+                self.film_bufs["pos_" + c] = dev_arr(pos)
+            for k in ("U_abs", "window_sum", "peak"):
+                self.film_bufs[k] = dev_zeros(n * 8)
+            fm = abi.Film()
+            fm.n, fm.dtype = n, self._ccode
+            for k, b in self.film_bufs.items():
+                setattr(fm, k, b.ptr)
+            self.film = fm
+            self.gd_arr = np.ascontiguousarray(spec.gd.as_array(np.float64))
+            tm, re_, T_min, inv_dT, nT = spec.lut
+            self.lut_tm, self.lut_re = dev_arr(tm), dev_arr(re_)
+            lut = abi.Lut()
+            lut.tm, lut.re, lut.T_min, lut.inv_dT, lut.n, lut.dtype = (
+                self.lut_tm.ptr, self.lut_re.ptr, T_min, inv_dT, nT, self._ccode)
+            self.lut = lut
+            self.Q_TM = float(spec.gd["Q_voigt_TM"])
+            self.Q_RE = float(spec.gd["Q_voigt_RE"])
+
+        # magneto-optic gyration
+        self.mo = None
+        if spec.mo is not None:
+            self._build_mo(spec.mo_poles)
+
+    # ---- construction helpers --------------------------------------------------------------
+    def _build_mo(self, mpoles: np.ndarray):
+        spec = self.spec
+        CT = spec.compute_dtype
+        n = len(spec.mo.idx)
+        np_ = len(mpoles) // 3
+        self.mo_poles_dev = DeviceBuffer.from_array(mpoles)
+        self._bufs.append(self.mo_poles_dev)
+        mb = {k: DeviceBuffer.zeros(n * np_ * CT.itemsize) for k in ("Pa", "Ja", "Pb", "Jb")}
+        mb.update(Ea_old=DeviceBuffer.zeros(n * CT.itemsize), Eb_old=DeviceBuffer.zeros(n * CT.itemsize),
+                  idx=DeviceBuffer.from_array(spec.mo.idx), fill=DeviceBuffer.from_array(spec.mo.fill),
+                  mpos=DeviceBuffer.from_array(spec.mo_pos))
+        self._bufs.extend(mb.values())
+        m = abi.Mo()
+        m.n, m.npoles, m.dtype = n, np_, self._ccode
+        m.Pa_from_b, m.Ja_from_b, m.Pb_from_a, m.Jb_from_a = mb["Pa"].ptr, mb["Ja"].ptr, mb["Pb"].ptr, mb["Jb"].ptr
+        m.E_old_a, m.E_old_b = mb["Ea_old"].ptr, mb["Eb_old"].ptr
+        m.idx, m.fill, m.mpos, m.poles = mb["idx"].ptr, mb["fill"].ptr, mb["mpos"].ptr, self.mo_poles_dev.ptr
+        self.mo = m
+        self.mo_bufs = mb
+        a, b = spec.mo_pair
+        self._mo_ea, self._mo_eb = self.f["e" + a], self.f["e" + b]
+        ci = {"x": 0, "y": 1, "z": 2}
+        self._mo_iea, self._mo_ieb = self.inv_eps[ci[a]], self.inv_eps[ci[b]]
+
+    # ---- the step (Solver.jl:283-369) --------------------------------------------------------
+    def _inject(self):
+        src = self.spec.source
+        if src is None:
+            return
+        y = C.byref(self.yee)
+        if isinstance(src, SoftSource):
+            i, j, k = src.index
+            _lib.call("inject_soft_3d", y, COMPONENT_CODE[src.component], i, j, k, src.pulse.value(self.t))
+        elif isinstance(src, ModeSource):
+            _lib.call("inject_mode_x_3d", y, COMPONENT_CODE[src.component], self._mode_i,
+                      self._mode_profile.vp, src.pulse.value(self.t), self._mode_hcorr)
+        elif isinstance(src, PlaneSource):
+            g = self.spec.grid
+            axis = {"x": g.x, "y": g.y, "z": g.z}[src.axis]
+            w = axis_weights(axis, src.position)
+            i0, w0 = w[0]
+            i1, w1 = w[1] if len(w) > 1 else (-1, 0.0)
+            _lib.call("inject_plane_3d", y, COMPONENT_CODE[src.component], AXIS_CODE[src.axis], i0, w0, i1,
+                      w1, src.pulse.value(self.t))
+        else:
+            raise TypeError(f"unsupported source {type(src).__name__}")
+
+    def step(self) -> "FDTDState":
+        spec = self.spec
+        y, cp = C.byref(self.yee), C.byref(self.cpml)
+        _lib.call("update_H_3d", y, cp, self.dt, spec.params.mu0)
+        self._inject()
+        _lib.call("update_E_3d", y, cp, self.dt)
+        if self.ade is not None:
+            for comp, a in zip(("ex", "ey", "ez"), self.ade):
+                _lib.call("patch_E_dispersive", self.f[comp].vp, C.byref(a), self._fcode, self.dt)
+        if self.mo is not None and self.film is not None:
+            _lib.call("patch_E_mo_gyration", self._mo_ea.vp, self._mo_eb.vp, C.byref(self.mo),
+                      self._mo_iea.vp, self._mo_ieb.vp, self._fcode, self.film_bufs["m_TM_x"].vp,
+                      self.film_bufs["m_RE_x"].vp, self.Q_TM, self.Q_RE, self.dt)
+        self.n += 1
+        self.t += self.dt
+        if self.film is not None:
+            hw = 1 if spec.has_ade_work() else 0
+            ax = C.byref(self.ade[0]) if hw else None
+            ay = C.byref(self.ade[1]) if hw else None
+            az = C.byref(self.ade[2]) if hw else None
+            _lib.call("pabs_accumulate", C.byref(self.film), y, ax, ay, az, hw, spec.pabs_factor(), self.dt)
+            if (self.multiphysics_enabled and spec.multiphysics_every > 0
+                    and self.n % spec.multiphysics_every == 0):
+                self._consume_window()
+        return self
+
+    def _consume_window(self):
+        """_consume_multiphysics_window! (Solver.jl:373-383)."""
+        window = self.n - self.last_mp_n
+        if window <= 0:
+            return
+        dt_mp = self.dt * window
+        _lib.call("multiphysics_step", C.byref(self.film), self.gd_arr.ctypes.data_as(C.c_void_p),
+                  C.byref(self.lut), dt_mp, self.spec.subcycles, self.spec.brillouin_iters, 1)
+        self.last_mp_n = self.n
+
+    def flush_multiphysics(self):
+        """flush_multiphysics! (Solver.jl:387-390)."""
+        if self.multiphysics_enabled and self.film is not None:
+            self._consume_window()
+        return self
+
+    def freeze_multiphysics(self):
+        self.multiphysics_enabled = False
+        return self
+
+    def enable_multiphysics(self):
+        self.multiphysics_enabled = True
+        self.last_mp_n = self.n
+        return self
+
+    def relax_step(self, dt: float, subcycles: Optional[int] = None):
+        """relax_step! (Solver.jl:548-555): 4TM+LLB with no heating."""
+        if self.film is None:
+            return self
+        _lib.call("multiphysics_step", C.byref(self.film), self.gd_arr.ctypes.data_as(C.c_void_p),
+                  C.byref(self.lut), float(dt), int(subcycles or self.spec.subcycles),
+                  self.spec.brillouin_iters, 0)
+        self.t += float(dt)
+        return self
+
+    def run(self, steps: int, callback=None):
+        """run! (Solver.jl:557-563)."""
+        for _ in range(int(steps)):
+            self.step()
+            if callback is not None:
+                callback(self)
+        return self
+
+    # ---- host access -------------------------------------------------------------------------
+    def synchronize(self):
+        _lib.call("synchronize")
+
+    def field(self, name: str) -> np.ndarray:
+        """One field volume as a host (nx, ny, nz) array (to_host)."""
+        nx, ny, nz = self.spec.shape
+        return self.f[name.lower()].download(self.spec.field_dtype).reshape((nx, ny, nz), order="F")
+
+    def film_state(self) -> dict:
+        if self.film is None:
+            return {}
+        out = {k: self.film_bufs[k].download(self.spec.compute_dtype) for k in FILM_NAMES}
+        for k in ("U_abs", "window_sum", "peak"):
+            out[k] = self.film_bufs[k].download(np.float64)
+        return out
+
+    def magnetization_snapshot(self) -> dict:
+        """magnetization_snapshot (Solver.jl:448-467)."""
+        fs = self.film_state()
+        out = {k: fs.get(k, np.zeros(0)).astype(np.float64) for k in FILM_NAMES[4:]}
+        out["material_cells"] = self.spec.film_cells.copy()
+        return out
+
+    def apply_magnetization(self, data: dict):
+        """apply_magnetization! (Solver.jl:474-483)."""
+        if self.film is None:
+            raise ValueError("state has no MagnetizationState")
+        for key in FILM_NAMES[4:]:
+            src = np.asarray(data[key], dtype=np.float64).astype(self.spec.compute_dtype)
+            if len(src) != self.spec.n_film:
+                raise ValueError(f"magnetization {key} length mismatch")
+            self.film_bufs[key].upload(src)
+        return self
+
+    def set_film(self, key: str, values: np.ndarray):
+        self.film_bufs[key].upload(np.asarray(values).astype(self.spec.compute_dtype))
+
+    def ade_state(self, comp: int) -> dict:
+        sd = self.spec.ade_state_dtype
+        b = self.ade_bufs[comp]
+        dt = {"f32": np.float32, "f64": np.float64, "bf16": np.uint16, "f16": np.float16}[sd]
+        return dict(P=b["P"].download(dt), J=b["J"].download(dt), E_old=b["E_old"].download(self.spec.compute_dtype))
+
+    def configure_probe_mode(self, lambda0: float = 532e-9, reset_ade: bool = True,
+                             freeze_magnetization: bool = True, enable_magneto_optic: Optional[bool] = None):
+        """configure_probe_mode! (Solver.jl:416-443): refit the poles at the probe
+        wavelength, zero the ADE / MO histories and freeze the magnetisation."""
+        from .grid import FDTDParams
+        if self.film is None:
+            return self
+        spec = self.spec
+        gd = spec.gd
+        p = FDTDParams()
+        CT = spec.compute_dtype
+        if enable_magneto_optic is None:
+            enable_magneto_optic = self.mo is not None
+        if self.ade is not None:
+            dp = dispersion.poles_array(dispersion.build_probe_poles(self.dt, p.eps0, gd), CT)
+            if len(dp) != len(spec.diag_poles):
+                raise ValueError("probe pole set must have the pump set's pole count")
+            self.poles_dev.upload(dp)
+        if enable_magneto_optic:
+            mp_ = dispersion.poles_array(dispersion.build_probe_mo_poles(self.dt, p.eps0, gd), CT)
+            if self.mo is None:
+                if spec.ade is None:
+                    raise ValueError("MO gyration needs the per-component lists")
+                from .spec import ADEList, position_map
+                comp_b = {"x": 0, "y": 1, "z": 2}[spec.mo_pair[1]]
+                base = spec.ade[comp_b]
+                spec.mo = ADEList(base.idx.copy(), base.fill.copy(), base.inv.copy())
+                spec.mo_pos = position_map(spec.mo.idx, spec.film_cells)
+                self._build_mo(mp_)
+            else:
+                self.mo_poles_dev.upload(mp_)
+        else:
+            self.mo = None
+        if reset_ade:
+            if self.ade_bufs is not None:
+                for b in self.ade_bufs:
+                    for k in ("P", "J", "E_old"):
+                        _lib.call("memset", b[k].vp, 0, C.c_size_t(b[k].nbytes))
+            if self.mo is not None:
+                for k in ("Pa", "Ja", "Pb", "Jb", "Ea_old", "Eb_old"):
+                    b = self.mo_bufs[k]
+                    _lib.call("memset", b.vp, 0, C.c_size_t(b.nbytes))
+        if freeze_magnetization:
+            self.freeze_multiphysics()
+        return self
+
+    def free(self):
+        """free_state! (Solver.jl:521-546)."""
+        for b in self._bufs:
+            b.free()
+        for d in (getattr(self, "film_bufs", {}), getattr(self, "mo_bufs", {})):
+            for b in d.values():
+                b.free()
+        for b in (getattr(self, "lut_tm", None), getattr(self, "lut_re", None)):
+            if b is not None:
+                b.free()
+        self._bufs = []

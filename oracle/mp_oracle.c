@@ -1,0 +1,435 @@
+/*
+ * ORACLE -- test infrastructure, not product code.
+ *
+ * CPU restatement of MagnetoPhotonic.jl's coupled 3-D time step kernels (the reference
+ * executes the same kernel bodies on its CPU backend through KernelAbstractions).  It is
+ * the checker for libmphip: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  Its entry points mirror include/mphip.h with an `orc_`
+ * prefix and take HOST pointers.
+ *
+ * Reference files followed (paths relative to the reference repository):
+ *   src/fdtd/Kernels.jl:9-152    Yee H/E + CFS-CPML           -> orc_yee.h
+ *   src/fdtd/Kernels.jl:154-355  sources, ADE, MO, pabs       -> orc_film.h
+ *   src/fdtd/Kernels.jl:362-412  multiphysics (4TM + LLB)     -> orc_physics.h
+ *   src/physics/Thermal.jl:24-40, src/physics/Magnetization.jl:7-263
+ *   src/fdtd/Kernels.jl:420-478  DFT phase/accumulate, trace  -> below
+ *   src/fdtd/Source.jl:188-236   plane-source host loop        -> below
+ *   src/sim/Monitors.jl:263-341  FluxMonitor plane integral    -> below
+ *
+ * Built with -O2 -ffp-contract=off -fno-fast-math -fopenmp (denormals kept).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/mphip.h"
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+#include "orc_physics.h"
+
+/* ---- Yee instantiations: (ST, CT, PT) ------------------------------------------------- */
+#define ST float
+#define CT float
+#define PT float
+#define SFX(n) n##_fff
+#define CFMA fmaf
+#include "orc_yee.h"
+#undef ST
+#undef CT
+#undef PT
+#undef SFX
+#undef CFMA
+
+#define ST double
+#define CT double
+#define PT double
+#define SFX(n) n##_ddd
+#define CFMA fma
+#include "orc_yee.h"
+#undef ST
+#undef CT
+#undef PT
+#undef SFX
+#undef CFMA
+
+#define ST float
+#define CT double
+#define PT float
+#define SFX(n) n##_fdf
+#define CFMA fma
+#include "orc_yee.h"
+#undef ST
+#undef CT
+#undef PT
+#undef SFX
+#undef CFMA
+
+#define ST double
+#define CT double
+#define PT float
+#define SFX(n) n##_ddf
+#define CFMA fma
+#include "orc_yee.h"
+#undef ST
+#undef CT
+#undef PT
+#undef SFX
+#undef CFMA
+
+/* ---- film / source instantiations: (ST, CT) -------------------------------------------- */
+#define ST float
+#define CT float
+#define SFX2(n) n##_ff
+#define CFMA fmaf
+#include "orc_film.h"
+#undef ST
+#undef CT
+#undef SFX2
+#undef CFMA
+
+#define ST double
+#define CT double
+#define SFX2(n) n##_dd
+#define CFMA fma
+#include "orc_film.h"
+#undef ST
+#undef CT
+#undef SFX2
+#undef CFMA
+
+#define ST float
+#define CT double
+#define SFX2(n) n##_fd
+#define CFMA fma
+#include "orc_film.h"
+#undef ST
+#undef CT
+#undef SFX2
+#undef CFMA
+
+static _Thread_local char orc_err[256];
+static int orc_fail(const char *msg) {
+    strncpy(orc_err, msg, sizeof(orc_err) - 1);
+    return MP_ERR_ARG;
+}
+ORC_EXPORT const char *orc_last_error(void) { return orc_err; }
+
+/* storage/compute pair code */
+static int pair_code(int32_t st, int32_t ct) {
+    if (st == MP_F32 && ct == MP_F32) return 0;
+    if (st == MP_F64 && ct == MP_F64) return 1;
+    if (st == MP_F32 && ct == MP_F64) return 2;
+    return -1;
+}
+
+ORC_EXPORT int orc_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0) {
+    if (g->nx < 2 || g->ny < 2 || g->nz < 2) return 0;
+    int pc = pair_code(g->field_dtype, g->compute_dtype);
+    if (pc == 0 && c->psi_dtype == MP_F32) {
+        orc_yee_H_fff(g, c, (float)dt / (float)mu0);
+    } else if (pc == 1 && c->psi_dtype == MP_F64) {
+        orc_yee_H_ddd(g, c, dt / mu0);
+    } else if (pc == 2 && c->psi_dtype == MP_F32) {
+        orc_yee_H_fdf(g, c, dt / mu0);
+    } else if (pc == 1 && c->psi_dtype == MP_F32) {
+        orc_yee_H_ddf(g, c, dt / mu0);
+    } else {
+        return orc_fail("orc_update_H_3d: unsupported dtype combination");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_update_E_3d(const mp_yee *g, mp_cpml *c, double dt) {
+    if (g->nx < 2 || g->ny < 2 || g->nz < 2) return 0;
+    int pc = pair_code(g->field_dtype, g->compute_dtype);
+    if (pc == 0 && c->psi_dtype == MP_F32) {
+        orc_yee_E_fff(g, c, (float)dt);
+    } else if (pc == 1 && c->psi_dtype == MP_F64) {
+        orc_yee_E_ddd(g, c, dt);
+    } else if (pc == 2 && c->psi_dtype == MP_F32) {
+        orc_yee_E_fdf(g, c, dt);
+    } else if (pc == 1 && c->psi_dtype == MP_F32) {
+        orc_yee_E_ddf(g, c, dt);
+    } else {
+        return orc_fail("orc_update_E_3d: unsupported dtype combination");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_inject_soft_3d(const mp_yee *g, int32_t component, int64_t i, int64_t j,
+                                  int64_t k, double value) {
+    if (i < 0 || j < 0 || k < 0 || i >= g->nx || j >= g->ny || k >= g->nz)
+        return orc_fail("orc_inject_soft_3d: index out of range");
+    const int64_t o = i + g->nx * (j + g->ny * k);
+    switch (pair_code(g->field_dtype, g->compute_dtype)) {
+    case 0: orc_inject_soft_ff(g, component, o, (float)value); break;
+    case 1: orc_inject_soft_dd(g, component, o, value); break;
+    case 2: orc_inject_soft_fd(g, component, o, value); break;
+    default: return orc_fail("orc_inject_soft_3d: unsupported dtypes");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_inject_mode_x_3d(const mp_yee *g, int32_t component, int64_t i,
+                                    const void *profile, double value, double hcorr) {
+    if (i < 0 || i >= g->nx) return orc_fail("orc_inject_mode_x_3d: index out of range");
+    switch (pair_code(g->field_dtype, g->compute_dtype)) {
+    case 0: orc_inject_mode_x_ff(g, component, i, (const float *)profile, (float)value, (float)hcorr); break;
+    case 1: orc_inject_mode_x_dd(g, component, i, (const double *)profile, value, hcorr); break;
+    case 2: orc_inject_mode_x_fd(g, component, i, (const double *)profile, value, hcorr); break;
+    default: return orc_fail("orc_inject_mode_x_3d: unsupported dtypes");
+    }
+    return 0;
+}
+
+/* inject!(::FieldState, ::Grid3D, ::PlaneSource) and _inject_3d_cell! (Source.jl:188-236):
+ * fields.E[i,j,k] += w*val promotes to Float64 and rounds once into the stored type;
+ * D += Float64(w*val) / Float64(inv_eps). */
+static void plane_cell(const mp_yee *g, int code, int64_t o, double dv) {
+    int f64 = g->field_dtype == MP_F64;
+#define ADD(ptr, inc)                                                          \
+    do {                                                                       \
+        if (f64) ((double *)(ptr))[o] = ((double *)(ptr))[o] + (inc);          \
+        else ((float *)(ptr))[o] = (float)((double)((float *)(ptr))[o] + (inc)); \
+    } while (0)
+#define IE(ptr) (f64 ? ((const double *)(ptr))[o] : (double)((const float *)(ptr))[o])
+    switch (code) {
+    case 1: ADD(g->ex, dv); ADD(g->dx, dv / IE(g->inv_eps_x)); break;
+    case 2: ADD(g->ey, dv); ADD(g->dy, dv / IE(g->inv_eps_y)); break;
+    case 3: ADD(g->ez, dv); ADD(g->dz, dv / IE(g->inv_eps_z)); break;
+    case 4: ADD(g->hx, dv); break;
+    case 5: ADD(g->hy, dv); break;
+    case 6: ADD(g->hz, dv); break;
+    default: break;
+    }
+#undef ADD
+#undef IE
+}
+
+ORC_EXPORT int orc_inject_plane_3d(const mp_yee *g, int32_t component, int32_t axis, int64_t i0,
+                                   double w0, int64_t i1, double w1, double value) {
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int64_t n_ax = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    if (axis < 0 || axis > 2 || i0 < 0 || i0 >= n_ax || i1 >= n_ax)
+        return orc_fail("orc_inject_plane_3d: bad axis or index");
+    int64_t idxs[2] = {i0, i1};
+    double ws[2] = {w0, w1};
+    /* Julia loop order: for (i,wi) in weights, j, k (x axis) etc.; each cell touched by
+     * exactly one weight, order irrelevant when i0 != i1. */
+    for (int t = 0; t < 2; ++t) {
+        if (idxs[t] < 0) continue;
+        double dv = ws[t] * value;
+        for (int64_t k = 0; k < nz; ++k)
+            for (int64_t j = 0; j < ny; ++j)
+                for (int64_t i = 0; i < nx; ++i) {
+                    int64_t c = axis == 0 ? i : (axis == 1 ? j : k);
+                    if (c != idxs[t]) continue;
+                    plane_cell(g, component, i + nx * (j + ny * k), dv);
+                }
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_patch_E_dispersive(void *E, mp_ade *s, int32_t field_dtype, double dt) {
+    if (s->n <= 0 || s->npoles <= 0) return 0;
+    switch (pair_code(field_dtype, s->dtype)) {
+    case 0: orc_ade_patch_ff((float *)E, s, 2.0f / (float)dt); break;
+    case 1: orc_ade_patch_dd((double *)E, s, 2.0 / dt); break;
+    case 2: orc_ade_patch_fd((float *)E, s, 2.0 / dt); break;
+    default: return orc_fail("orc_patch_E_dispersive: unsupported dtypes");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_patch_E_mo_gyration(void *Ea, void *Eb, mp_mo *s, const void *iea,
+                                       const void *ieb, int32_t field_dtype, const void *m_TM,
+                                       const void *m_RE, double Q_TM, double Q_RE, double dt) {
+    if (s->n <= 0 || s->npoles <= 0) return 0;
+    switch (pair_code(field_dtype, s->dtype)) {
+    case 0:
+        orc_mo_gyration_ff((float *)Ea, (float *)Eb, s, (const float *)iea, (const float *)ieb,
+                           (const float *)m_TM, (const float *)m_RE, (float)Q_TM, (float)Q_RE,
+                           2.0f / (float)dt);
+        break;
+    case 1:
+        orc_mo_gyration_dd((double *)Ea, (double *)Eb, s, (const double *)iea, (const double *)ieb,
+                           (const double *)m_TM, (const double *)m_RE, Q_TM, Q_RE, 2.0 / dt);
+        break;
+    case 2:
+        orc_mo_gyration_fd((float *)Ea, (float *)Eb, s, (const float *)iea, (const float *)ieb,
+                           (const double *)m_TM, (const double *)m_RE, Q_TM, Q_RE, 2.0 / dt);
+        break;
+    default: return orc_fail("orc_patch_E_mo_gyration: unsupported dtypes");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
+                                   const mp_ade *az, int32_t has_ade_work, double pabs_factor,
+                                   double dt) {
+    if (f->n <= 0) return 0;
+    int ct = has_ade_work ? ax->dtype : f->dtype;
+    switch (pair_code(g->field_dtype, ct)) {
+    case 0: orc_pabs_ff(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
+    case 1: orc_pabs_dd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
+    case 2: orc_pabs_fd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
+    default: return orc_fail("orc_pabs_accumulate: unsupported dtypes");
+    }
+    return 0;
+}
+
+ORC_EXPORT int orc_multiphysics_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
+                                     double dt, int32_t subcycles, int32_t brillouin_iters,
+                                     int32_t has_pabs) {
+    if (f->n <= 0) return 0;
+    if (subcycles <= 0) return orc_fail("orc_multiphysics_step: subcycles must be positive");
+    if (f->dtype == MP_F32) {
+        orc_multiphysics_float(f, p, (const float *)lut->tm, (const float *)lut->re,
+                               (double)(float)lut->T_min, (double)(float)lut->inv_dT, lut->n,
+                               (double)(float)dt, subcycles, brillouin_iters, has_pabs);
+    } else if (f->dtype == MP_F64) {
+        orc_multiphysics_double(f, p, (const double *)lut->tm, (const double *)lut->re,
+                                lut->T_min, lut->inv_dT, lut->n, dt, subcycles, brillouin_iters,
+                                has_pabs);
+    } else {
+        return orc_fail("orc_multiphysics_step: unsupported dtype");
+    }
+    return 0;
+}
+
+/* Single-cell llb_step and update_4tm for the known-answer tests (Magnetization.jl:105,
+ * Thermal.jl:33).  m[6] in/out, out[4] = dm_TM_x, dm_RE_x, Hani_TM_x, Hani_RE_x. */
+ORC_EXPORT void orc_llb_step(double *m, double te, double tsT, double tsR, const mp_ferri_params *p,
+                             const double *tm_lut, const double *re_lut, double T_min,
+                             double inv_dT, int32_t lut_N, double dt, int32_t iters, double *out) {
+    orc_llb_step_double(m, te, tsT, tsR, p, tm_lut, re_lut, T_min, inv_dT, lut_N, dt, iters, out);
+}
+ORC_EXPORT void orc_update_4tm_cell(double *T4, double pabs, double QT, double QR,
+                                    const mp_ferri_params *p, double dt) {
+    orc_update_4tm(&T4[0], &T4[1], &T4[2], &T4[3], pabs, QT, QR, p, dt);
+}
+ORC_EXPORT double orc_brillouin_fn(double S, double x) { return orc_brillouin(S, x); }
+ORC_EXPORT double orc_lut_lookup(const double *lut, double T, double T_min, double inv_dT,
+                                 int32_t n) {
+    return orc_lut_lookup_double(lut, T, T_min, inv_dT, n);
+}
+
+/* ---- monitors ------------------------------------------------------------------------ */
+static inline double load_f(const void *p, int32_t dt, int64_t o) {
+    return dt == MP_F64 ? ((const double *)p)[o] : (double)((const float *)p)[o];
+}
+
+/* plane geometry: normal axis -> in-plane sizes and the tangential (a, b) components */
+static void plane_geom(const mp_yee *g, int axis, int64_t *n1, int64_t *n2, const void **Ea,
+                       const void **Eb, const void **Ha, const void **Hb) {
+    if (axis == 0) {
+        *n1 = g->ny; *n2 = g->nz;
+        *Ea = g->ey; *Eb = g->ez; *Ha = g->hy; *Hb = g->hz;
+    } else if (axis == 1) {
+        *n1 = g->nz; *n2 = g->nx;
+        *Ea = g->ez; *Eb = g->ex; *Ha = g->hz; *Hb = g->hx;
+    } else {
+        *n1 = g->nx; *n2 = g->ny;
+        *Ea = g->ex; *Eb = g->ey; *Ha = g->hx; *Hb = g->hy;
+    }
+}
+
+static inline int64_t plane_lin(const mp_yee *g, int axis, int64_t idx, int64_t a, int64_t b) {
+    /* (a, b) are the in-plane coordinates in plane order */
+    if (axis == 0) return idx + g->nx * (a + g->ny * b);        /* (j, k) */
+    if (axis == 1) return b + g->nx * (idx + g->ny * a);        /* (k, i) */
+    return a + g->nx * (b + g->ny * idx);                       /* (i, j) */
+}
+
+/* probe_phase_factors_kernel! + probe_dft_accumulate_kernel! (Kernels.jl:420-461) */
+ORC_EXPORT int orc_probe_dft_accumulate(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
+                                        const double *omega_bins, double *phase, int32_t nb,
+                                        double t, double dt) {
+    for (int q = 0; q < nb; ++q) {
+        double ph = omega_bins[q] * t;
+        phase[2 * q] = cos(ph) * dt;
+        phase[2 * q + 1] = -sin(ph) * dt;
+    }
+    for (int p = 0; p < nplanes; ++p) {
+        const mp_plane *pl = &planes[p];
+        int64_t n1, n2;
+        const void *Ea, *Eb, *Ha, *Hb;
+        plane_geom(g, pl->axis, &n1, &n2, &Ea, &Eb, &Ha, &Hb);
+        const int64_t np = n1 * n2;
+        for (int64_t b = 0; b < n2; ++b)
+            for (int64_t a = 0; a < n1; ++a) {
+                const int64_t o = plane_lin(g, pl->axis, pl->index, a, b);
+                const int64_t c = a + n1 * b;
+                double ea = load_f(Ea, g->field_dtype, o), eb = load_f(Eb, g->field_dtype, o);
+                double ha = load_f(Ha, g->field_dtype, o), hb = load_f(Hb, g->field_dtype, o);
+                pl->energy[c] += (ea * hb - eb * ha) * dt;
+                for (int q = 0; q < nb; ++q) {
+                    const double pr = phase[2 * q], pi = phase[2 * q + 1];
+                    const int64_t z = 2 * (c + np * q);
+                    pl->dft_ea[z] += ea * pr; pl->dft_ea[z + 1] += ea * pi;
+                    pl->dft_eb[z] += eb * pr; pl->dft_eb[z + 1] += eb * pi;
+                    pl->dft_ha[z] += ha * pr; pl->dft_ha[z + 1] += ha * pi;
+                    pl->dft_hb[z] += hb * pr; pl->dft_hb[z + 1] += hb * pi;
+                }
+            }
+    }
+    return 0;
+}
+
+/* probe_trace_plane_kernel! + sum(dims=(1,2)) (Kernels.jl:467-478; ProbeReadout.jl:286-291) */
+ORC_EXPORT int orc_probe_trace_plane(const mp_yee *g, int32_t axis, int64_t i_pre, int64_t i_post,
+                                     const double *mode_w, const double *area, double *out) {
+    int64_t n1, n2;
+    const void *Ea, *Eb, *Ha, *Hb;
+    plane_geom(g, axis, &n1, &n2, &Ea, &Eb, &Ha, &Hb);
+    double s[4] = {0, 0, 0, 0};
+    for (int64_t b = 0; b < n2; ++b)
+        for (int64_t a = 0; a < n1; ++a) {
+            const int64_t c = a + n1 * b;
+            double w = mode_w[c] * area[c];
+            int64_t o1 = plane_lin(g, axis, i_pre, a, b), o2 = plane_lin(g, axis, i_post, a, b);
+            s[0] += load_f(Ea, g->field_dtype, o1) * w;
+            s[1] += load_f(Eb, g->field_dtype, o1) * w;
+            s[2] += load_f(Ea, g->field_dtype, o2) * w;
+            s[3] += load_f(Eb, g->field_dtype, o2) * w;
+        }
+    memcpy(out, s, sizeof(s));
+    return 0;
+}
+
+/* record!(::FluxMonitor) 3-D branch (Monitors.jl:302-339) */
+ORC_EXPORT int orc_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const double *area,
+                              double *out, int64_t slot) {
+    int64_t n1, n2;
+    const void *Ea, *Eb, *Ha, *Hb;
+    plane_geom(g, axis, &n1, &n2, &Ea, &Eb, &Ha, &Hb);
+    double flux = 0.0;
+    for (int64_t b = 0; b < n2; ++b)
+        for (int64_t a = 0; a < n1; ++a) {
+            const int64_t o = plane_lin(g, axis, index, a, b);
+            flux += (load_f(Ea, g->field_dtype, o) * load_f(Hb, g->field_dtype, o) -
+                     load_f(Eb, g->field_dtype, o) * load_f(Ha, g->field_dtype, o)) *
+                    area[a + n1 * b];
+        }
+    out[slot] = flux;
+    return 0;
+}
+
+ORC_EXPORT int orc_film_reduce(const mp_film *f, double *out) {
+    double s[8] = {0};
+    double neg = 0.0, temax = -INFINITY;
+    const void *arr[8] = {f->m_TM_x, f->m_TM_y, f->m_TM_z, f->m_RE_x, f->m_RE_y, f->m_RE_z, f->Te, f->Tl};
+    for (int64_t n = 0; n < f->n; ++n) {
+        for (int a = 0; a < 8; ++a) s[a] += load_f(arr[a], f->dtype, n);
+        if (load_f(f->m_TM_x, f->dtype, n) < 0.0) neg += 1.0;
+        double te = load_f(f->Te, f->dtype, n);
+        if (te > temax) temax = te;
+    }
+    double inv = f->n > 0 ? 1.0 / (double)f->n : 0.0;
+    for (int a = 0; a < 8; ++a) out[a] = s[a] * inv;
+    out[8] = neg * inv;
+    out[9] = f->n > 0 ? temax : 0.0;
+    return 0;
+}

@@ -1,0 +1,282 @@
+/*
+ * ORACLE (test infrastructure only).  FP64 restatement of the 4TM + two-sublattice LLB
+ * integrator: update_Te_energy / update_4tm (src/physics/Thermal.jl:24-40), brillouin,
+ * lookup_m_eq_lut, hot_weight, damp_radial_growth, llb_step
+ * (src/physics/Magnetization.jl:7-22,75-99,105-263).  Expressions keep Julia's
+ * left-to-right evaluation order; no contraction (-ffp-contract=off).
+ */
+#ifndef ORC_PHYSICS_H
+#define ORC_PHYSICS_H
+
+/* Julia's max/min for floats (NaN-propagating, -0.0 < 0.0), and clamp
+ * (clamp(x, lo, hi) = x > hi ? hi : (x < lo ? lo : x)). */
+static inline double jl_max(double x, double y) {
+    if ((y > x) || (signbit(y) < signbit(x))) return isnan(x) ? x : y;
+    return isnan(y) ? y : x;
+}
+static inline double jl_min(double x, double y) {
+    if ((y < x) || (signbit(y) > signbit(x))) return isnan(x) ? x : y;
+    return isnan(y) ? y : x;
+}
+static inline double jl_clamp(double x, double lo, double hi) {
+    return x > hi ? hi : (x < lo ? lo : x);
+}
+
+/* Magnetization.jl:7-22 */
+static double orc_brillouin(double S, double x) {
+    if (!isfinite(x)) return 0.0;
+    double a = (2.0 * S + 1.0) / (2.0 * S);
+    double b = 1.0 / (2.0 * S);
+    if (fabs(x) < 1e-4) {
+        double a2 = a * a, b2 = b * b;
+        /* a^4 for a literal integer power: Julia's power-by-squaring, (a*a)*(a*a). */
+        double a4 = a2 * a2, b4 = b2 * b2;
+        return ((a * a - b * b) / 3.0) * x - ((a4 - b4) / 45.0) * (x * x * x);
+    }
+    double ax = a * x;
+    double bx = b * x;
+    double coth_ax = fabs(ax) > 500.0 ? copysign(1.0, ax) : (exp(2.0 * ax) + 1.0) / (exp(2.0 * ax) - 1.0);
+    double coth_bx = fabs(bx) > 500.0 ? copysign(1.0, bx) : (exp(2.0 * bx) + 1.0) / (exp(2.0 * bx) - 1.0);
+    return jl_clamp(a * coth_ax - b * coth_bx, -1.0, 1.0);
+}
+
+/* Magnetization.jl:75-86 */
+#define ORC_LUT_LOOKUP(LT)                                                               \
+    static double orc_lut_lookup_##LT(const LT *lut, double T, double T_min, double inv_dT, \
+                                      int32_t lut_N) {                                   \
+        double pos = (T - T_min) * inv_dT;                                               \
+        if (!isfinite(pos)) return 0.0;                                                  \
+        double pos_c = jl_clamp(pos, 0.0, (double)(lut_N - 1));                          \
+        double base = floor(pos_c);                                                      \
+        int64_t i0 = (int64_t)base + 1;                                                  \
+        if (i0 < 1) i0 = 1;                                                              \
+        if (i0 > lut_N) i0 = lut_N;                                                      \
+        int64_t i1 = i0 + 1 < (int64_t)lut_N ? i0 + 1 : (int64_t)lut_N;                  \
+        double frac = jl_clamp(pos_c - base, 0.0, 1.0);                                  \
+        return (1.0 - frac) * (double)lut[i0 - 1] + frac * (double)lut[i1 - 1];          \
+    }
+ORC_LUT_LOOKUP(float)
+ORC_LUT_LOOKUP(double)
+
+/* Magnetization.jl:88 */
+static inline double orc_hot_weight(double T, double Tsw, double inv_dTsw) {
+    return 1.0 / (1.0 + exp(-(T - Tsw) * inv_dTsw));
+}
+
+/* Magnetization.jl:92-99 */
+static inline void orc_damp_radial(double mx, double my, double mz, double *dx, double *dy,
+                                   double *dz, double mag, double limit, double width) {
+    double radial_dot = mx * *dx + my * *dy + mz * *dz;
+    if (radial_dot > 0.0 && mag > limit - width) {
+        double scale = jl_clamp((limit - mag) / width, 0.0, 1.0);
+        *dx = *dx * scale;
+        *dy = *dy * scale;
+        *dz = *dz * scale;
+    }
+}
+
+/* Thermal.jl:24-29 */
+static inline double orc_update_Te_energy(double te, double tl, double tsTM, double tsRE,
+                                          double pabs, const mp_ferri_params *gd, double dt) {
+    double ue = 0.5 * gd->gamma_e * (te * te);
+    double q = -gd->Gel * (te - tl) - gd->Ges_TM * (te - tsTM) - gd->Ges_RE * (te - tsRE) + pabs;
+    double ue_new = jl_max(0.5 * gd->gamma_e, ue + dt * q);
+    return sqrt(2.0 * ue_new / gd->gamma_e);
+}
+
+/* Thermal.jl:33-40 */
+static inline void orc_update_4tm(double *te, double *tl, double *tsT, double *tsR, double pabs,
+                                  double QT, double QR, const mp_ferri_params *gd, double dt) {
+    const double tau_th = 2.0e-12;
+    double te0 = *te, tl0 = *tl, t0 = *tsT, r0 = *tsR;
+    double te_new = orc_update_Te_energy(te0, tl0, t0, r0, pabs, gd, dt);
+    double tl_new = tl0 + dt * (gd->Gel * (te0 - tl0) * gd->inv_Cl) - dt * (tl0 - gd->T0) / tau_th;
+    double tsT_new = t0 + dt * (gd->Ges_TM * (te0 - t0) - QT) * gd->inv_Cs_TM;
+    double tsR_new = r0 + dt * (gd->Ges_RE * (te0 - r0) - QR) * gd->inv_Cs_RE;
+    *te = te_new;
+    *tl = tl_new;
+    *tsT = jl_max(1.0, tsT_new);
+    *tsR = jl_max(1.0, tsR_new);
+}
+
+/* llb_step (Magnetization.jl:105-263).  m[0..5] = TM xyz, RE xyz, updated in place;
+ * out[0..3] = dm_TM_x, dm_RE_x, Hani_TM_x, Hani_RE_x. */
+#define ORC_LLB(LT)                                                                          \
+static void orc_llb_step_##LT(double *m, double te_current, double ts_TM_old, double ts_RE_old,  \
+                              const mp_ferri_params *gd, const LT *tm_lut, const LT *re_lut,  \
+                              double lut_T_min, double lut_inv_dT, int32_t lut_N, double dt,   \
+                              int brillouin_iters, double *out) {                              \
+    double mx_TM = m[0], my_TM = m[1], mz_TM = m[2], mx_RE = m[3], my_RE = m[4], mz_RE = m[5]; \
+    double ts_avg = 0.5 * (ts_TM_old + ts_RE_old);                                           \
+    double ratio_TM = ts_TM_old * gd->inv_T_Curie;                                           \
+    double ratio_RE = ts_RE_old * gd->inv_T_Curie;                                           \
+    double m_eq_TM = orc_lut_lookup_##LT(tm_lut, ts_TM_old, lut_T_min, lut_inv_dT, lut_N);    \
+    double m_eq_RE = orc_lut_lookup_##LT(re_lut, ts_RE_old, lut_T_min, lut_inv_dT, lut_N);    \
+    double chi_par_TM = ratio_TM < 1.0 ? gd->alpha0_TM / (gd->T_Curie * jl_max(1.0 - ratio_TM, 0.01)) \
+                                       : gd->alpha0_TM / (gd->T_Curie * jl_max(ratio_TM - 1.0, 0.01)); \
+    double chi_par_RE = ratio_RE < 1.0 ? gd->alpha0_RE / (gd->T_Curie * jl_max(1.0 - ratio_RE, 0.01)) \
+                                       : gd->alpha0_RE / (gd->T_Curie * jl_max(ratio_RE - 1.0, 0.01)); \
+    double alpha_par_TM = ratio_TM < 1.0 ? gd->alpha0_TM * 2.0 * jl_max(ratio_TM, 0.01) / 3.0 : gd->alpha0_TM; \
+    double alpha_perp_TM = ratio_TM < 1.0 ? gd->alpha0_TM * (1.0 - jl_max(ratio_TM, 0.01) / 3.0) : gd->alpha0_TM; \
+    double alpha_par_RE = ratio_RE < 1.0 ? gd->alpha0_RE * 2.0 * jl_max(ratio_RE, 0.01) / 3.0 : gd->alpha0_RE; \
+    double alpha_perp_RE = ratio_RE < 1.0 ? gd->alpha0_RE * (1.0 - jl_max(ratio_RE, 0.01) / 3.0) : gd->alpha0_RE; \
+    double m2_TM = mx_TM * mx_TM + my_TM * my_TM + mz_TM * mz_TM;                            \
+    double m2_RE = mx_RE * mx_RE + my_RE * my_RE + mz_RE * mz_RE;                            \
+    double mag_TM = sqrt(m2_TM);                                                             \
+    double mag_RE = sqrt(m2_RE);                                                             \
+    double exchange_softening = jl_max(0.35, 300.0 / jl_max(te_current, 300.0));             \
+    double J_eff_TMRE = gd->J_TMRE * exchange_softening;                                     \
+    double hot_window = 0.0;                                                                 \
+    if (te_current > gd->T_Curie) {                                                          \
+        double u = (te_current - gd->Tsw) * (1.0 / gd->dTsw);                               \
+        hot_window = exp(-0.5 * (u * u));                                                    \
+    }                                                                                        \
+    double Hsw = gd->Hsw0 * hot_window;                                                      \
+    double twelve_J_eff = 12.0 * J_eff_TMRE;                                                 \
+    double inv_kb_ts_TM = 1.0 / (gd->kb * jl_max(ts_TM_old, 1.0));                           \
+    double inv_kb_ts_RE = 1.0 / (gd->kb * jl_max(ts_RE_old, 1.0));                           \
+    double c_TM = twelve_J_eff * gd->inv_mu_TM;                                              \
+    double c_RE = twelve_J_eff * gd->inv_mu_RE;                                              \
+    double Hex_TM_x = c_TM * mx_RE, Hex_TM_y = c_TM * my_RE, Hex_TM_z = c_TM * mz_RE;        \
+    double Hex_RE_x = c_RE * mx_TM, Hex_RE_y = c_RE * my_TM, Hex_RE_z = c_RE * mz_TM;        \
+    double Hani_TM_x = gd->two_K_over_Ms_TM * mx_TM;                                         \
+    double Hani_RE_x = gd->two_K_over_Ms_RE * mx_RE;                                         \
+    double Hx_TM = Hex_TM_x + Hani_TM_x - Hsw, Hy_TM = Hex_TM_y, Hz_TM = Hex_TM_z;           \
+    double Hx_RE = Hex_RE_x + Hani_RE_x + Hsw, Hy_RE = Hex_RE_y, Hz_RE = Hex_RE_z;           \
+    double Hlong_fac_TM, Hlong_fac_RE;                                                       \
+    if (ts_avg < gd->T_Curie) {                                                              \
+        for (int it = 0; it < brillouin_iters; ++it) {                                       \
+            double x_TM = (gd->twelve_J_TMTM * m_eq_TM + twelve_J_eff * m_eq_RE) * inv_kb_ts_TM; \
+            double x_RE = (gd->twelve_J_RERE * m_eq_RE + twelve_J_eff * m_eq_TM) * inv_kb_ts_RE; \
+            m_eq_TM = jl_clamp(orc_brillouin(gd->S_TM, x_TM), -1.0, 1.0);                    \
+            m_eq_RE = jl_clamp(orc_brillouin(gd->S_RE, x_RE), -1.0, 1.0);                    \
+        }                                                                                    \
+        const double m_eq_floor = 0.01;                                                      \
+        Hlong_fac_TM = (fabs(m_eq_TM) - mag_TM) / (2.0 * chi_par_TM * jl_max(fabs(m_eq_TM), m_eq_floor)); \
+        Hlong_fac_RE = (fabs(m_eq_RE) - mag_RE) / (2.0 * chi_par_RE * jl_max(fabs(m_eq_RE), m_eq_floor)); \
+    } else {                                                                                 \
+        Hlong_fac_TM = -mag_TM / (2.0 * chi_par_TM);                                         \
+        Hlong_fac_RE = -mag_RE / (2.0 * chi_par_RE);                                         \
+    }                                                                                        \
+    double mdotH_TM = mx_TM * Hx_TM + my_TM * Hy_TM + mz_TM * Hz_TM;                         \
+    double mdotH_RE = mx_RE * Hx_RE + my_RE * Hy_RE + mz_RE * Hz_RE;                         \
+    double cx_TM = my_TM * Hz_TM - mz_TM * Hy_TM;                                            \
+    double cy_TM = mz_TM * Hx_TM - mx_TM * Hz_TM;                                            \
+    double cz_TM = mx_TM * Hy_TM - my_TM * Hx_TM;                                            \
+    double cx_RE = my_RE * Hz_RE - mz_RE * Hy_RE;                                            \
+    double cy_RE = mz_RE * Hx_RE - mx_RE * Hz_RE;                                            \
+    double cz_RE = mx_RE * Hy_RE - my_RE * Hx_RE;                                            \
+    double ccx_TM = my_TM * cz_TM - mz_TM * cy_TM;                                           \
+    double ccy_TM = mz_TM * cx_TM - mx_TM * cz_TM;                                           \
+    double ccz_TM = mx_TM * cy_TM - my_TM * cx_TM;                                           \
+    double ccx_RE = my_RE * cz_RE - mz_RE * cy_RE;                                           \
+    double ccy_RE = mz_RE * cx_RE - mx_RE * cz_RE;                                           \
+    double ccz_RE = mx_RE * cy_RE - my_RE * cx_RE;                                           \
+    double inv_m2_TM = 1.0 / (m2_TM + 1e-30);                                                \
+    double inv_m2_RE = 1.0 / (m2_RE + 1e-30);                                                \
+    double g = gd->gamma_gyro;                                                               \
+    double Lambda_TM = jl_clamp(g * Hlong_fac_TM, -1.0 / 0.12e-12, 1.0 / 2.80e-12);          \
+    double Lambda_RE = jl_clamp(g * Hlong_fac_RE, -1.0 / 0.45e-12, 1.0 / 3.20e-12);          \
+    double dm_TM_x_trans = -g * cx_TM + g * alpha_par_TM * inv_m2_TM * mdotH_TM * mx_TM - g * alpha_perp_TM * inv_m2_TM * ccx_TM; \
+    double dm_TM_y_trans = -g * cy_TM + g * alpha_par_TM * inv_m2_TM * mdotH_TM * my_TM - g * alpha_perp_TM * inv_m2_TM * ccy_TM; \
+    double dm_TM_z_trans = -g * cz_TM + g * alpha_par_TM * inv_m2_TM * mdotH_TM * mz_TM - g * alpha_perp_TM * inv_m2_TM * ccz_TM; \
+    double dm_RE_x_trans = -g * cx_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * mx_RE - g * alpha_perp_RE * inv_m2_RE * ccx_RE; \
+    double dm_RE_y_trans = -g * cy_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * my_RE - g * alpha_perp_RE * inv_m2_RE * ccy_RE; \
+    double dm_RE_z_trans = -g * cz_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * mz_RE - g * alpha_perp_RE * inv_m2_RE * ccz_RE; \
+    double W_TM_scatter = orc_hot_weight(te_current, 0.92 * gd->T_Curie, 1.0 / 40.0);       \
+    double W_RE_scatter = orc_hot_weight(ts_RE_old, 0.95 * gd->T_Curie, 1.0 / 75.0);        \
+    double cool_TM = 1.0 - orc_hot_weight(ts_TM_old, 0.90 * gd->T_Curie, 1.0 / 55.0);       \
+    double cool_RE = 1.0 - orc_hot_weight(ts_RE_old, 0.88 * gd->T_Curie, 1.0 / 75.0);       \
+    double tm_switched_gate = orc_hot_weight(-mx_TM, 0.006, 1.0 / 0.010);                    \
+    double re_res_floor = 0.28 * W_TM_scatter * (1.0 - 0.50 * W_RE_scatter);                 \
+    double tm_cross_floor = 0.16 * W_TM_scatter;                                             \
+    double tm_target_mag = 0.50 + 0.49 * cool_TM;                                            \
+    double tm_target_x_pre = -jl_max(jl_max(fabs(mx_RE), re_res_floor), tm_cross_floor);     \
+    double tm_target_x_post = -tm_target_mag;                                                \
+    double tm_target_x = (1.0 - tm_switched_gate) * tm_target_x_pre + tm_switched_gate * tm_target_x_post; \
+    double rate_TM_pre = 10.0e12 * W_TM_scatter;                                             \
+    double rate_TM_post = 1.15e12 * W_TM_scatter + 0.24e12 * cool_TM;                        \
+    double rate_TM = (1.0 - tm_switched_gate) * rate_TM_pre + tm_switched_gate * rate_TM_post; \
+    double re_branch_gate = tm_switched_gate * W_RE_scatter;                                 \
+    double re_target_mag = 0.18 + 0.80 * cool_RE;                                            \
+    double re_target_x = (1.0 - re_branch_gate) * mx_RE + re_branch_gate * re_target_mag;    \
+    double rate_RE = 0.45e12 * W_RE_scatter + 0.25e12 * cool_RE;                             \
+    double dm_TM_x = dm_TM_x_trans + rate_TM * (tm_target_x - mx_TM);                        \
+    double dm_TM_y = dm_TM_y_trans - rate_TM * my_TM;                                        \
+    double dm_TM_z = dm_TM_z_trans - rate_TM * mz_TM;                                        \
+    double dm_RE_x = dm_RE_x_trans + rate_RE * (re_target_x - mx_RE);                        \
+    double dm_RE_y = dm_RE_y_trans - rate_RE * my_RE;                                        \
+    double dm_RE_z = dm_RE_z_trans - rate_RE * mz_RE;                                        \
+    orc_damp_radial(mx_TM, my_TM, mz_TM, &dm_TM_x, &dm_TM_y, &dm_TM_z, mag_TM, 0.990, 0.55); \
+    orc_damp_radial(mx_RE, my_RE, mz_RE, &dm_RE_x, &dm_RE_y, &dm_RE_z, mag_RE, 0.968, 0.50); \
+    double lambda_TM_eff = Lambda_TM;                                                        \
+    double lambda_RE_eff = Lambda_RE;                                                        \
+    if (mx_TM < -0.020 && lambda_TM_eff < 0.0) lambda_TM_eff = 0.0;                          \
+    if (lambda_TM_eff > 0.0) lambda_TM_eff *= jl_clamp((0.990 - mag_TM) / 0.55, 0.0, 1.0);   \
+    if (lambda_RE_eff > 0.0) lambda_RE_eff *= jl_clamp((0.968 - mag_RE) / 0.50, 0.0, 1.0);   \
+    double exp_TM = exp(jl_clamp(lambda_TM_eff * dt, -20.0, 20.0));                          \
+    double exp_RE = exp(jl_clamp(lambda_RE_eff * dt, -20.0, 20.0));                          \
+    double m_TM_x_new = mx_TM * exp_TM + dt * dm_TM_x;                                       \
+    double m_TM_y_new = my_TM * exp_TM + dt * dm_TM_y;                                       \
+    double m_TM_z_new = mz_TM * exp_TM + dt * dm_TM_z;                                       \
+    double m_RE_x_new = mx_RE * exp_RE + dt * dm_RE_x;                                       \
+    double m_RE_y_new = my_RE * exp_RE + dt * dm_RE_y;                                       \
+    double m_RE_z_new = mz_RE * exp_RE + dt * dm_RE_z;                                       \
+    double mag_new_TM = sqrt(m_TM_x_new * m_TM_x_new + m_TM_y_new * m_TM_y_new + m_TM_z_new * m_TM_z_new); \
+    if (mag_new_TM > 0.992) {                                                                \
+        double s = 0.992 / mag_new_TM;                                                       \
+        m_TM_x_new *= s; m_TM_y_new *= s; m_TM_z_new *= s;                                   \
+    }                                                                                        \
+    double mag_new_RE = sqrt(m_RE_x_new * m_RE_x_new + m_RE_y_new * m_RE_y_new + m_RE_z_new * m_RE_z_new); \
+    if (mag_new_RE > 0.972) {                                                                \
+        double s = 0.972 / mag_new_RE;                                                       \
+        m_RE_x_new *= s; m_RE_y_new *= s; m_RE_z_new *= s;                                   \
+    }                                                                                        \
+    out[0] = m_TM_x_new - mx_TM;                                                             \
+    out[1] = m_RE_x_new - mx_RE;                                                             \
+    out[2] = Hani_TM_x;                                                                      \
+    out[3] = Hani_RE_x;                                                                      \
+    m[0] = m_TM_x_new; m[1] = m_TM_y_new; m[2] = m_TM_z_new;                                 \
+    m[3] = m_RE_x_new; m[4] = m_RE_y_new; m[5] = m_RE_z_new;                                 \
+}
+ORC_LLB(float)
+ORC_LLB(double)
+
+/* multiphysics_kernel! (Kernels.jl:362-412) for film storage LT (= CT).  dt, T_min and
+ * inv_dT arrive already rounded to CT by the caller, as the launcher does. */
+#define ORC_MP(LT)                                                                           \
+static void orc_multiphysics_##LT(mp_film *f, const mp_ferri_params *gd, const LT *tm_lut,   \
+                                  const LT *re_lut, double T_min, double inv_dT, int32_t lut_N, \
+                                  double dt, int subcycles, int brillouin_iters, int has_pabs) { \
+    LT *Te = (LT *)f->Te, *Tl = (LT *)f->Tl, *TsT = (LT *)f->Ts_TM, *TsR = (LT *)f->Ts_RE;   \
+    LT *mTx = (LT *)f->m_TM_x, *mTy = (LT *)f->m_TM_y, *mTz = (LT *)f->m_TM_z;               \
+    LT *mRx = (LT *)f->m_RE_x, *mRy = (LT *)f->m_RE_y, *mRz = (LT *)f->m_RE_z;               \
+    const int64_t N = f->n;                                                                  \
+    _Pragma("omp parallel for schedule(static)")                                             \
+    for (int64_t n = 0; n < N; ++n) {                                                        \
+        double pabs = 0.0;                                                                   \
+        if (has_pabs) {                                                                      \
+            pabs = f->window_sum[n] / dt;                                                    \
+            f->window_sum[n] = 0.0;                                                          \
+        }                                                                                    \
+        double te = (double)Te[n], tl = (double)Tl[n], tsT = (double)TsT[n], tsR = (double)TsR[n]; \
+        double m[6] = {(double)mTx[n], (double)mTy[n], (double)mTz[n],                       \
+                       (double)mRx[n], (double)mRy[n], (double)mRz[n]};                      \
+        double sub_dt = dt / (double)subcycles;                                              \
+        for (int s = 0; s < subcycles; ++s) {                                                \
+            double out[4];                                                                   \
+            orc_llb_step_##LT(m, te, tsT, tsR, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, \
+                              brillouin_iters, out);                                         \
+            double Q_spin_TM = gd->Ms_TM * out[2] * (out[0] / sub_dt);                       \
+            double Q_spin_RE = gd->Ms_RE * out[3] * (out[1] / sub_dt);                       \
+            orc_update_4tm(&te, &tl, &tsT, &tsR, pabs, Q_spin_TM, Q_spin_RE, gd, sub_dt);    \
+        }                                                                                    \
+        Te[n] = (LT)te; Tl[n] = (LT)tl; TsT[n] = (LT)tsT; TsR[n] = (LT)tsR;                  \
+        mTx[n] = (LT)m[0]; mTy[n] = (LT)m[1]; mTz[n] = (LT)m[2];                             \
+        mRx[n] = (LT)m[3]; mRy[n] = (LT)m[4]; mRz[n] = (LT)m[5];                             \
+    }                                                                                        \
+}
+ORC_MP(float)
+ORC_MP(double)
+
+#endif
