@@ -1,0 +1,149 @@
+"""Benchmark of the coupled 3-D time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload coupled_pump|em_stencil]
+                    [--form plain|mixed] [--no-cpu]
+
+A step is one pass of the full coupled sequence (Yee H, pump source, Yee E, ADE x/y/z,
+gyrotropic MO, absorbed power, 4TM + LLB with 4 substeps, DFT on 6 planes x 4 bins)
+over the 512 x 512 x 256 synthetic pump workload, inputs resident in HBM.  Prints one
+JSON line: Mcell-updates/s = Nx*Ny*Nz*K / wall, the roofline of the dominant kernel
+(algorithmic bytes per launch / mean launch time from HIP events recorded around every
+launch in the timed region) and the CPU oracle timed on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "magnetophotonic.jl_amd"))
+
+METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Δm|≤1e-3"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "multiphysics",
+           "dft_phase", "dft_accumulate")
+
+
+def build_workload(name: str, form: str):
+    from mphip import configs
+    if name == "coupled_pump":
+        kw = {}
+        if form == "mixed":
+            kw = dict(psi_precision="bf16", ade_state_precision="bf16")
+        spec, meta = configs.coupled_pump(nx=512, ny=512, nz=256, steps=20000, **kw)
+    elif name == "em_stencil":
+        spec, meta = configs.em_stencil(n=384, npml=16, steps=5000)
+        meta["planes"], meta["omega_bins"] = [], []
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return spec, meta
+
+
+def cpu_baseline(spec, meta, steps: int, warmup: int = 1):
+    """The C oracle (a restatement of the reference CPU path) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import driver
+    from mphip import abi
+    L = driver.lib(abi)
+    L.orc_num_threads.restype = __import__("ctypes").c_int
+    st = driver.OracleState(spec)
+    for _ in range(warmup):
+        st.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st.step()
+    el = time.perf_counter() - t0
+    nx, ny, nz = spec.shape
+    return dict(value=round(nx * ny * nz * steps / el / 1e6, 2), unit="Mcell-updates/s",
+                cores=int(L.orc_num_threads()), kind="port",
+                sample=f"{steps} steps of {meta['name']} after {warmup} warm-up step(s), FP32, "
+                       f"C oracle with OpenMP, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="coupled_pump")
+    ap.add_argument("--form", default="plain", choices=("plain", "mixed"))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=24)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(the coupled step would shard on z with per-step halo planes, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    from mphip import _lib, roofline
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+
+    spec, meta = build_workload(args.workload, args.form)
+    planes, bins = meta.get("planes", []), meta.get("omega_bins", [])
+    st = FDTDState(spec)
+    if planes:
+        st.attach_dft(DFTPlanes(st, planes, bins))
+    L = _lib.lib()
+    st.run(args.warmup)
+    st.synchronize()
+
+    # timed region: K steps, per-launch HIP events on the launch stream
+    L.mp_profile_enable(1)
+    st.synchronize()
+    t0 = time.perf_counter()
+    st.run(args.steps)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    stats = {}
+    for k in KERNELS:
+        n, ms = C.c_int64(), C.c_double()
+        L.mp_kernel_stats(k.encode(), C.byref(n), C.byref(ms))
+        if n.value:
+            stats[k] = dict(launches=n.value, avg_us=round(ms.value / n.value * 1e3, 2))
+    L.mp_profile_enable(0)
+    nx, ny, nz = spec.shape
+    ncell = nx * ny * nz
+    mcell = ncell * args.steps / wall / 1e6
+    kb = roofline.step_bytes(spec, planes, len(bins))
+    dom = max((k for k in stats if k in kb), key=lambda k: stats[k]["avg_us"] * stats[k]["launches"])
+    per_launch = kb[dom] / (stats[dom]["launches"] / args.steps)
+    achieved = per_launch / (stats[dom]["avg_us"] * 1e-6) / 1e9
+    for k, v in stats.items():
+        if k in kb:
+            launches_per_step = v["launches"] / args.steps
+            v["alg_GBs"] = round(kb[k] / launches_per_step / (v["avg_us"] * 1e-6) / 1e9, 1)
+    step_gbs = kb["total"] / (wall / args.steps) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(mcell, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.form == "plain" else "f32 (bf16 psi/ADE storage)",
+        "data": "synthetic",
+        "config": {"workload": meta["name"], "grid": list(spec.shape), "film_cells": spec.n_film,
+                   "form": args.form, "dft_planes": len(planes), "dft_bins": len(bins),
+                   "llb_substeps": spec.subcycles, "cpml": list(spec.cpml.widths)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": int(per_launch)},
+        "step_roofline": {"alg_bytes_per_step": kb["total"],
+                          "alg_bytes_per_cell": round(kb["total"] / ncell, 2),
+                          "achieved_GBs": round(step_gbs, 1), "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "kernels": stats,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(spec, meta, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

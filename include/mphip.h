@@ -174,6 +174,55 @@ typedef struct mp_plane {
 
 #define MP_MAX_PLANES 8
 
+/* Source waveform evaluated on the host in FP64 once per step (src/fdtd/Source.jl:1-33). */
+enum mp_pulse_kind {
+    MP_PULSE_GAUSSIAN = 1,          /* A exp(-((t-t0)/tau)^2) sin(w (t-t0) + phase)  */
+    MP_PULSE_CARRIER_GAUSSIAN = 2,  /* A sin(w t) exp(-1/2 ((t-t0)/tau)^2)           */
+    MP_PULSE_CONTINUOUS = 3         /* A sin(w t + phase)                             */
+};
+typedef struct mp_pulse {
+    int32_t kind;
+    int32_t _pad;
+    double amplitude, t0, tau, omega, phase;
+} mp_pulse;
+
+/* Source of a state (Solver.jl:296-311): soft point, plane (any axis, two weights) or
+ * x-normal mode plane. */
+enum mp_source_kind { MP_SRC_NONE = 0, MP_SRC_SOFT = 1, MP_SRC_PLANE = 2, MP_SRC_MODE_X = 3 };
+typedef struct mp_source {
+    int32_t kind, component, axis, _pad;
+    int64_t i, j, k;        /* soft: the cell; mode: plane index in i */
+    int64_t i0, i1;         /* plane: cells along `axis` (i1 = -1 for one) */
+    double w0, w1;
+    const void *profile;    /* mode: (ny, nz) CT */
+    double hcorr;
+    mp_pulse pulse;
+} mp_source;
+
+/* Everything one step of an FDTDState touches (Solver.jl:9-53).  Pointers to the
+ * component structs above; NULL disables that part (no dispersion, no MO, no film, no
+ * monitor).  t, n and last_mp_n are advanced by mp_run_steps. */
+typedef struct mp_state {
+    mp_yee *yee;
+    mp_cpml *cpml;
+    mp_ade *ade[3];
+    mp_mo *mo;
+    void *mo_ea, *mo_eb;
+    const void *mo_iea, *mo_ieb;
+    mp_film *film;
+    const mp_ferri_params *params;
+    const mp_lut *lut;
+    mp_source source;
+    const mp_plane *planes;
+    int32_t nplanes, nb;
+    const double *omega_bins;
+    double *phase;
+    double dt, mu0, pabs_factor, Q_TM, Q_RE, t;
+    int64_t n, last_mp_n;
+    int32_t multiphysics_every, subcycles, brillouin_iters, multiphysics_enabled;
+    int32_t has_ade_work, record_dft;
+} mp_state;
+
 /* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */
 
 /* Select the device and create the library state.  Backend.jl:19-33 (__init__ /
@@ -292,6 +341,16 @@ int mp_probe_trace_plane(const mp_yee *g, int32_t axis, int64_t i_pre, int64_t i
  * (src/sim/Monitors.jl:263-341) with an on-device reduction. */
 int mp_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const double *area,
                   double *out, int64_t slot);
+
+/* nsteps of the fixed step sequence, all launches on the library stream, no host sync:
+ * H -> source(t) -> E -> ADE x,y,z -> MO -> n,t += 1,dt -> pabs -> 4TM+LLB every
+ * multiphysics_every steps over the window since the last consume -> DFT record.
+ * Replaces step!/_step_typed! + run! (src/fdtd/Solver.jl:277-383, 557-563) and the
+ * per-step record! of ProbeReadout (src/sim/ProbeReadout.jl:303-307). */
+int mp_run_steps(mp_state *s, int64_t nsteps);
+
+/* Value of a pulse at time t (FP64), as the step driver evaluates it. */
+double mp_pulse_value(const mp_pulse *p, double t);
 
 /* Film reductions on device (src/sim/Monitors.jl:487-578 FilmAverage/SwitchedFraction,
  * src/physics/Magnetization.jl:280-283 norms): out[0..9] = means of m_TM_xyz, m_RE_xyz,

@@ -1,0 +1,92 @@
+// Host-side step driver: the fixed per-step launch sequence of an FDTDState, issued
+// from C so a time loop costs no interpreter round trips.
+//
+// Reference: step!/_step_typed! (src/fdtd/Solver.jl:277-369), the multiphysics window
+// _consume_multiphysics_window! (:373-383), run! (:557-563), the pulse values of
+// src/fdtd/Source.jl:1-33 and the per-step device record! of ProbeReadout
+// (src/sim/ProbeReadout.jl:274-307).
+#include <cmath>
+
+#include "common.h"
+
+using namespace mp;
+
+MP_API double mp_pulse_value(const mp_pulse *p, double t) {
+    switch (p->kind) {
+    case MP_PULSE_GAUSSIAN: {
+        const double u = (t - p->t0) / p->tau;
+        return p->amplitude * std::exp(-(u * u)) * std::sin(p->omega * (t - p->t0) + p->phase);
+    }
+    case MP_PULSE_CARRIER_GAUSSIAN: {
+        const double u = (t - p->t0) / p->tau;
+        return p->amplitude * std::sin(p->omega * t) * std::exp(-0.5 * (u * u));
+    }
+    case MP_PULSE_CONTINUOUS:
+        return p->amplitude * std::sin(p->omega * t + p->phase);
+    default:
+        return 0.0;
+    }
+}
+
+static int inject(mp_state *s) {
+    const mp_source &src = s->source;
+    if (src.kind == MP_SRC_NONE) return MP_OK;
+    const double v = mp_pulse_value(&src.pulse, s->t);
+    switch (src.kind) {
+    case MP_SRC_SOFT: return mp_inject_soft_3d(s->yee, src.component, src.i, src.j, src.k, v);
+    case MP_SRC_PLANE: return mp_inject_plane_3d(s->yee, src.component, src.axis, src.i0, src.w0, src.i1, src.w1, v);
+    case MP_SRC_MODE_X: return mp_inject_mode_x_3d(s->yee, src.component, src.i, src.profile, v, src.hcorr);
+    default: return fail_arg("mp_run_steps: unknown source kind %d", src.kind);
+    }
+}
+
+static int one_step(mp_state *s) {
+    int rc;
+    if ((rc = mp_update_H_3d(s->yee, s->cpml, s->dt, s->mu0))) return rc;
+    if ((rc = inject(s))) return rc;
+    if ((rc = mp_update_E_3d(s->yee, s->cpml, s->dt))) return rc;
+    if (s->ade[0]) {
+        void *E[3] = {s->yee->ex, s->yee->ey, s->yee->ez};
+        for (int c = 0; c < 3; ++c)
+            if ((rc = mp_patch_E_dispersive(E[c], s->ade[c], s->yee->field_dtype, s->dt))) return rc;
+    }
+    if (s->mo && s->film) {
+        if ((rc = mp_patch_E_mo_gyration(s->mo_ea, s->mo_eb, s->mo, s->mo_iea, s->mo_ieb,
+                                         s->yee->field_dtype, s->film->m_TM_x, s->film->m_RE_x, s->Q_TM,
+                                         s->Q_RE, s->dt)))
+            return rc;
+    }
+    s->n += 1;
+    s->t += s->dt;
+    if (s->film) {
+        if ((rc = mp_pabs_accumulate(s->film, s->yee, s->ade[0], s->ade[1], s->ade[2], s->has_ade_work,
+                                     s->pabs_factor, s->dt)))
+            return rc;
+        if (s->multiphysics_enabled && s->multiphysics_every > 0 && s->n % s->multiphysics_every == 0) {
+            const int64_t window = s->n - s->last_mp_n;
+            if (window > 0) {
+                if ((rc = mp_multiphysics_step(s->film, s->params, s->lut, s->dt * (double)window, s->subcycles,
+                                               s->brillouin_iters, 1)))
+                    return rc;
+                s->last_mp_n = s->n;
+            }
+        }
+    }
+    if (s->record_dft && s->nplanes > 0) {
+        if ((rc = mp_probe_dft_accumulate(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t,
+                                          s->dt)))
+            return rc;
+    }
+    return MP_OK;
+}
+
+MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
+    if (!s || !s->yee || !s->cpml) return fail_arg("mp_run_steps: null state");
+    if (s->film && (!s->params || !s->lut)) return fail_arg("mp_run_steps: film state needs params and LUT");
+    if (s->ade[0] && (!s->ade[1] || !s->ade[2])) return fail_arg("mp_run_steps: ADE needs all three lists");
+    for (int64_t k = 0; k < nsteps; ++k) {
+        int rc = one_step(s);
+        if (rc) return rc;
+    }
+    return MP_OK;
+}

@@ -30,6 +30,7 @@ def lib():
             raise MPError(f"libmphip.so not found at {LIB_PATH}; build it with __graft_entry__.build()")
         h = C.CDLL(LIB_PATH)
         abi.bind(h, "mp_", abi.SIGNATURES)
+        abi.bind(h, "mp_", abi.DRIVER_SIGNATURES)
         abi.bind(h, "", abi.RUNTIME_SIGNATURES)
         _lib = h
     return _lib

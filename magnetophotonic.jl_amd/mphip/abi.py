@@ -62,6 +62,31 @@ class Plane(C.Structure):
                 ("dft_ha", vp), ("dft_hb", vp), ("energy", vp)]
 
 
+class Pulse(C.Structure):
+    _fields_ = [("kind", i32), ("_pad", i32), ("amplitude", f64), ("t0", f64), ("tau", f64),
+                ("omega", f64), ("phase", f64)]
+
+
+class Source(C.Structure):
+    _fields_ = [("kind", i32), ("component", i32), ("axis", i32), ("_pad", i32),
+                ("i", i64), ("j", i64), ("k", i64), ("i0", i64), ("i1", i64), ("w0", f64), ("w1", f64),
+                ("profile", vp), ("hcorr", f64), ("pulse", Pulse)]
+
+
+class State(C.Structure):
+    _fields_ = [("yee", C.POINTER(Yee)), ("cpml", C.POINTER(Cpml)), ("ade", C.POINTER(Ade) * 3),
+                ("mo", C.POINTER(Mo)), ("mo_ea", vp), ("mo_eb", vp), ("mo_iea", vp), ("mo_ieb", vp),
+                ("film", C.POINTER(Film)), ("params", vp), ("lut", C.POINTER(Lut)), ("source", Source),
+                ("planes", C.POINTER(Plane)), ("nplanes", i32), ("nb", i32), ("omega_bins", vp),
+                ("phase", vp), ("dt", f64), ("mu0", f64), ("pabs_factor", f64), ("Q_TM", f64),
+                ("Q_RE", f64), ("t", f64), ("n", i64), ("last_mp_n", i64),
+                ("multiphysics_every", i32), ("subcycles", i32), ("brillouin_iters", i32),
+                ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32)]
+
+
+MP_PULSE_GAUSSIAN, MP_PULSE_CARRIER_GAUSSIAN, MP_PULSE_CONTINUOUS = 1, 2, 3
+MP_SRC_NONE, MP_SRC_SOFT, MP_SRC_PLANE, MP_SRC_MODE_X = 0, 1, 2, 3
+
 P = C.POINTER
 SIGNATURES = {
     "update_H_3d": (C.c_int, [P(Yee), P(Cpml), f64, f64]),
@@ -78,6 +103,11 @@ SIGNATURES = {
     "flux_plane": (C.c_int, [P(Yee), i32, i64, vp, vp, i64]),
     "film_reduce": (C.c_int, [P(Film), vp]),
     "last_error": (C.c_char_p, []),
+}
+
+DRIVER_SIGNATURES = {
+    "run_steps": (C.c_int, [P(State), i64]),
+    "pulse_value": (C.c_double, [P(Pulse), f64]),
 }
 
 RUNTIME_SIGNATURES = {

@@ -1,0 +1,76 @@
+"""Algorithmic (compulsory) HBM bytes of each kernel of the coupled step.
+
+Each array a kernel touches is counted once per read and once per write at its storage
+width; halo re-reads and cache effects are excluded.  These are the per-launch figures
+behind the roofline fractions the benchmark reports.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .cpml import PSI_AXIS
+
+_SZ = {"f32": 4, "f64": 8, "bf16": 2, "f16": 2}
+
+
+def _psi_cells(spec) -> int:
+    """Cells of the H (or E) update range that carry one psi term, summed over the 12
+    psi slabs: every slab cell is read and written once per step."""
+    nx, ny, nz = spec.shape
+    w = spec.cpml.widths
+    tot = 0
+    for s in range(12):
+        a = PSI_AXIS[s]
+        # each slab (lo or hi) holds w cells along its axis; the update range is n-1 wide
+        # along the other two axes
+        other = [(nx - 1), (ny - 1), (nz - 1)]
+        other[a] = w[a]
+        tot += other[0] * other[1] * other[2]
+    return tot
+
+
+def kernel_bytes(spec) -> dict:
+    """Algorithmic bytes per launch of each kernel of one step."""
+    nx, ny, nz = spec.shape
+    fs = spec.field_dtype.itemsize
+    cs = spec.compute_dtype.itemsize
+    ps = _SZ[spec.psi_dtype]
+    n_upd = (nx - 1) * (ny - 1) * (nz - 1)
+    out = {}
+    # H: read E(3), H(3); write H(3).  E: read H(3), D(3), inv_eps(3); write D(3), E(3).
+    out["yee_H"] = 9 * fs * n_upd + 2 * ps * _psi_cells(spec)
+    out["yee_E"] = 15 * fs * n_upd + 2 * ps * _psi_cells(spec)
+    if spec.ade is not None:
+        np_ = len(spec.diag_poles) // 3
+        ss = _SZ[spec.ade_state_dtype]
+        per = 4 + 2 * cs + 2 * cs + 2 * fs + 2 * 2 * np_ * ss      # idx, fill+inv, E_old r/w, E r/w, P,J r/w
+        out["ade_patch"] = sum(per * len(a.idx) for a in spec.ade)
+    if spec.mo is not None:
+        np_ = len(spec.mo_poles) // 3
+        per = 4 + 4 + cs + 2 * cs + 4 * fs + 4 * cs + 2 * fs + 4 * 2 * np_ * cs
+        out["mo_gyration"] = per * len(spec.mo.idx)
+    if spec.has_film:
+        nf = spec.n_film
+        np_ = (len(spec.diag_poles) // 3) if spec.ade is not None else 0
+        ss = _SZ[spec.ade_state_dtype]
+        work = (12 + 3 * np_ * ss) if spec.has_ade_work() else 0
+        out["pabs"] = nf * (4 + 3 * fs + work + 6 * 8)
+        out["multiphysics"] = nf * (20 * cs + 2 * 8)
+    return out
+
+
+def plane_bytes(spec, planes, nb: int) -> int:
+    """DFT record: per plane cell 4 tangential fields read, 4 complex accumulators read
+    and written per bin, the flux integral read and written."""
+    nx, ny, nz = spec.shape
+    fs = spec.field_dtype.itemsize
+    dims = {0: ny * nz, 1: nz * nx, 2: nx * ny}
+    return sum(dims[a] * (4 * fs + 4 * nb * 32 + 16) for a, _ in planes)
+
+
+def step_bytes(spec, planes=(), nb: int = 0) -> dict:
+    kb = kernel_bytes(spec)
+    if planes and nb:
+        kb["dft_accumulate"] = plane_bytes(spec, planes, nb)
+    kb["total"] = int(sum(v for v in kb.values()))
+    return kb

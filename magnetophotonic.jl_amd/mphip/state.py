@@ -15,8 +15,8 @@ from typing import Optional
 import numpy as np
 
 from . import _lib, abi, dispersion
-from .sources import (AXIS_CODE, COMPONENT_CODE, ModeSource, PlaneSource, SoftSource, axis_weights,
-                      source_index)
+from .sources import (AXIS_CODE, COMPONENT_CODE, CarrierGaussianPulse, ContinuousSource, GaussianPulse,
+                      ModeSource, PlaneSource, SoftSource, axis_weights, source_index)
 from .spec import StateSpec
 
 _DT_CODE = {np.dtype(np.float32): abi.MP_F32, np.dtype(np.float64): abi.MP_F64}
@@ -83,6 +83,8 @@ class FDTDState:
     def __init__(self, spec: StateSpec, device: int = 0):
         _lib.init(device)
         self.spec = spec
+        self._st = abi.State()
+        self._dft = None
         self.n = 0
         self.t = 0.0
         self.last_mp_n = 0
@@ -208,6 +210,7 @@ class FDTDState:
         self.mo = None
         if spec.mo is not None:
             self._build_mo(spec.mo_poles)
+        self._build_state()
 
     # ---- construction helpers --------------------------------------------------------------
     def _build_mo(self, mpoles: np.ndarray):
@@ -234,53 +237,143 @@ class FDTDState:
         ci = {"x": 0, "y": 1, "z": 2}
         self._mo_iea, self._mo_ieb = self.inv_eps[ci[a]], self.inv_eps[ci[b]]
 
-    # ---- the step (Solver.jl:283-369) --------------------------------------------------------
-    def _inject(self):
+    # ---- the step (Solver.jl:283-383) --------------------------------------------------------
+    # n, t, last_mp_n and the multiphysics flag live in the mp_state the C driver advances.
+    @property
+    def n(self) -> int:
+        return int(self._st.n)
+
+    @n.setter
+    def n(self, v):
+        self._st.n = int(v)
+
+    @property
+    def t(self) -> float:
+        return float(self._st.t)
+
+    @t.setter
+    def t(self, v):
+        self._st.t = float(v)
+
+    @property
+    def last_mp_n(self) -> int:
+        return int(self._st.last_mp_n)
+
+    @last_mp_n.setter
+    def last_mp_n(self, v):
+        self._st.last_mp_n = int(v)
+
+    @property
+    def multiphysics_enabled(self) -> bool:
+        return bool(self._st.multiphysics_enabled)
+
+    @multiphysics_enabled.setter
+    def multiphysics_enabled(self, v):
+        self._st.multiphysics_enabled = 1 if v else 0
+
+    def _source_struct(self) -> abi.Source:
         src = self.spec.source
+        s = abi.Source()
+        s.kind = abi.MP_SRC_NONE
         if src is None:
-            return
-        y = C.byref(self.yee)
+            return s
+        p = src.pulse
+        if isinstance(p, GaussianPulse):
+            s.pulse.kind, s.pulse.amplitude, s.pulse.t0 = abi.MP_PULSE_GAUSSIAN, p.amplitude, p.t0
+            s.pulse.tau, s.pulse.omega, s.pulse.phase = p.tau, p.omega, p.phase
+        elif isinstance(p, CarrierGaussianPulse):
+            s.pulse.kind, s.pulse.amplitude, s.pulse.t0 = abi.MP_PULSE_CARRIER_GAUSSIAN, p.amplitude, p.t0
+            s.pulse.tau, s.pulse.omega = p.sigma, p.omega
+        elif isinstance(p, ContinuousSource):
+            s.pulse.kind, s.pulse.amplitude, s.pulse.omega, s.pulse.phase = (
+                abi.MP_PULSE_CONTINUOUS, p.amplitude, p.omega, p.phase)
+        else:
+            raise TypeError(f"unsupported pulse {type(p).__name__}")
+        s.component = COMPONENT_CODE[src.component]
         if isinstance(src, SoftSource):
-            i, j, k = src.index
-            _lib.call("inject_soft_3d", y, COMPONENT_CODE[src.component], i, j, k, src.pulse.value(self.t))
+            s.kind = abi.MP_SRC_SOFT
+            s.i, s.j, s.k = (int(v) for v in src.index)
         elif isinstance(src, ModeSource):
-            _lib.call("inject_mode_x_3d", y, COMPONENT_CODE[src.component], self._mode_i,
-                      self._mode_profile.vp, src.pulse.value(self.t), self._mode_hcorr)
+            s.kind = abi.MP_SRC_MODE_X
+            s.i = self._mode_i
+            s.profile = self._mode_profile.ptr
+            s.hcorr = self._mode_hcorr
         elif isinstance(src, PlaneSource):
             g = self.spec.grid
-            axis = {"x": g.x, "y": g.y, "z": g.z}[src.axis]
-            w = axis_weights(axis, src.position)
-            i0, w0 = w[0]
-            i1, w1 = w[1] if len(w) > 1 else (-1, 0.0)
-            _lib.call("inject_plane_3d", y, COMPONENT_CODE[src.component], AXIS_CODE[src.axis], i0, w0, i1,
-                      w1, src.pulse.value(self.t))
+            w = axis_weights({"x": g.x, "y": g.y, "z": g.z}[src.axis], src.position)
+            s.kind = abi.MP_SRC_PLANE
+            s.axis = AXIS_CODE[src.axis]
+            s.i0, s.w0 = w[0]
+            s.i1, s.w1 = w[1] if len(w) > 1 else (-1, 0.0)
         else:
             raise TypeError(f"unsupported source {type(src).__name__}")
+        return s
+
+    def _build_state(self):
+        """Fill the mp_state the C step driver consumes (everything one step touches)."""
+        spec = self.spec
+        st = self._st
+        st.yee = C.pointer(self.yee)
+        st.cpml = C.pointer(self.cpml)
+        for c in range(3):
+            st.ade[c] = C.pointer(self.ade[c]) if self.ade is not None else None
+        if self.mo is not None and self.film is not None:
+            st.mo = C.pointer(self.mo)
+            st.mo_ea, st.mo_eb = self._mo_ea.ptr, self._mo_eb.ptr
+            st.mo_iea, st.mo_ieb = self._mo_iea.ptr, self._mo_ieb.ptr
+        else:
+            st.mo = None
+        if self.film is not None:
+            st.film = C.pointer(self.film)
+            st.params = self.gd_arr.ctypes.data
+            st.lut = C.pointer(self.lut)
+            st.Q_TM, st.Q_RE = self.Q_TM, self.Q_RE
+        st.source = self._source_struct()
+        st.dt, st.mu0, st.pabs_factor = self.dt, spec.params.mu0, spec.pabs_factor()
+        st.multiphysics_every, st.subcycles = spec.multiphysics_every, spec.subcycles
+        st.brillouin_iters = spec.brillouin_iters
+        st.has_ade_work = 1 if spec.has_ade_work() else 0
+
+    def set_source(self, src) -> None:
+        """Replace the state's source (the phase-source swap of run_phase!,
+        src/sim/Experiment.jl:254-290)."""
+        self.spec.source = src
+        if isinstance(src, ModeSource):
+            if src.axis != "x" or src.component not in ("Ey", "Ez"):
+                raise ValueError("ModeSource injection supports axis='x' with component Ey or Ez")
+            prof = np.asarray(src.profile, dtype=np.float64).reshape(-1, order="F").astype(self.spec.compute_dtype)
+            self._mode_profile = DeviceBuffer.from_array(prof)
+            self._bufs.append(self._mode_profile)
+            self._mode_i = source_index(self.spec.grid.x, src.position)
+            self._mode_hcorr = src.neff / math.sqrt(self.spec.params.mu0 / self.spec.params.eps0)
+        self._st.source = self._source_struct()
+
+    def attach_dft(self, mon) -> None:
+        """Record `mon` (monitors.DFTPlanes) inside every step of run()/step()."""
+        st = self._st
+        st.planes = C.cast(mon.arr, C.POINTER(abi.Plane))
+        st.nplanes, st.nb = len(mon.planes), mon.nb
+        st.omega_bins, st.phase = mon.omega_dev.ptr, mon.phase_dev.ptr
+        st.record_dft = 1
+        self._dft = mon
+
+    def detach_dft(self) -> None:
+        self._st.record_dft = 0
+        self._st.nplanes = 0
+        self._dft = None
 
     def step(self) -> "FDTDState":
-        spec = self.spec
-        y, cp = C.byref(self.yee), C.byref(self.cpml)
-        _lib.call("update_H_3d", y, cp, self.dt, spec.params.mu0)
-        self._inject()
-        _lib.call("update_E_3d", y, cp, self.dt)
-        if self.ade is not None:
-            for comp, a in zip(("ex", "ey", "ez"), self.ade):
-                _lib.call("patch_E_dispersive", self.f[comp].vp, C.byref(a), self._fcode, self.dt)
-        if self.mo is not None and self.film is not None:
-            _lib.call("patch_E_mo_gyration", self._mo_ea.vp, self._mo_eb.vp, C.byref(self.mo),
-                      self._mo_iea.vp, self._mo_ieb.vp, self._fcode, self.film_bufs["m_TM_x"].vp,
-                      self.film_bufs["m_RE_x"].vp, self.Q_TM, self.Q_RE, self.dt)
-        self.n += 1
-        self.t += self.dt
-        if self.film is not None:
-            hw = 1 if spec.has_ade_work() else 0
-            ax = C.byref(self.ade[0]) if hw else None
-            ay = C.byref(self.ade[1]) if hw else None
-            az = C.byref(self.ade[2]) if hw else None
-            _lib.call("pabs_accumulate", C.byref(self.film), y, ax, ay, az, hw, spec.pabs_factor(), self.dt)
-            if (self.multiphysics_enabled and spec.multiphysics_every > 0
-                    and self.n % spec.multiphysics_every == 0):
-                self._consume_window()
+        """One EM step (step!, Solver.jl:277-369)."""
+        return self.run(1)
+
+    def run(self, steps: int, callback=None):
+        """run! (Solver.jl:557-563): the whole step sequence in one C call per step batch."""
+        if callback is None:
+            _lib.call("run_steps", C.byref(self._st), int(steps))
+        else:
+            for _ in range(int(steps)):
+                _lib.call("run_steps", C.byref(self._st), 1)
+                callback(self)
         return self
 
     def _consume_window(self):
@@ -316,14 +409,6 @@ class FDTDState:
                   C.byref(self.lut), float(dt), int(subcycles or self.spec.subcycles),
                   self.spec.brillouin_iters, 0)
         self.t += float(dt)
-        return self
-
-    def run(self, steps: int, callback=None):
-        """run! (Solver.jl:557-563)."""
-        for _ in range(int(steps)):
-            self.step()
-            if callback is not None:
-                callback(self)
         return self
 
     # ---- host access -------------------------------------------------------------------------
@@ -403,6 +488,7 @@ class FDTDState:
                 self.mo_poles_dev.upload(mp_)
         else:
             self.mo = None
+        self._build_state()
         if reset_ade:
             if self.ade_bufs is not None:
                 for b in self.ade_bufs:

@@ -23,6 +23,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include "../include/mphip.h"
 
 #define ORC_EXPORT __attribute__((visibility("default")))
@@ -115,6 +117,7 @@ static int orc_fail(const char *msg) {
     return MP_ERR_ARG;
 }
 ORC_EXPORT const char *orc_last_error(void) { return orc_err; }
+ORC_EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
 
 /* storage/compute pair code */
 static int pair_code(int32_t st, int32_t ct) {
