@@ -13,6 +13,8 @@
 // Arithmetic order is the reference's: derivative = (E[n] - E[c]) * inv_d, psi update
 // = fma(b, psi, a*deriv) (Julia muladd), curl terms and the D/H updates exactly as
 // written there, compiled with -ffp-contract=off.
+#include <cstdlib>
+
 #include "common.h"
 
 namespace mp {
@@ -204,6 +206,233 @@ __global__ __launch_bounds__(TX *TY) void yee_E_kernel(
     }
 }
 
+// ---- FP32 vector form: 4 consecutive x cells per lane ---------------------------------
+// Each lane moves 16 B per access (dwordx4), which is what lets the stencil approach the
+// HBM copy rate; a wave covers 256 cells of one x row.  The x neighbour one past the
+// lane's group comes from the next lane by a cross-lane shuffle (lane 63 reads it from
+// memory).  Cells the reference does not update (H at i = nx-1, E at i = 0) are written
+// back unchanged.  Per-cell arithmetic is identical to the scalar kernels above.
+constexpr int VX = 4;
+
+__device__ __forceinline__ float4 ld4(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+__device__ __forceinline__ void st4(float *p, float4 v) { *reinterpret_cast<float4 *>(p) = v; }
+// Streamed-once arrays bypass cache retention so the L2 keeps the stencil's halo rows.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld4nt(const float *p) {
+    const v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4nt(float *p, float4 v) {
+    v4f w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(p));
+}
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each with
+// its own L2; remapping the dispatch index gives every XCD a contiguous range of tiles,
+// so a tile and its y neighbour (which shares a halo row) usually meet in the same L2.
+constexpr int NXCD = 8;
+struct Tile { int64_t bx, by, bz; };
+__device__ __forceinline__ Tile tile_of(int64_t nbx, int64_t nby, int64_t nbz, bool remap) {
+    const int64_t nb = nbx * nby * nbz;
+    const int64_t b = blockIdx.x;
+    int64_t t = b;
+    if (remap && nb % NXCD == 0) t = (b % NXCD) * (nb / NXCD) + b / NXCD;
+    Tile r;
+    r.bx = t % nbx;
+    r.by = (t / nbx) % nby;
+    r.bz = t / (nbx * nby);
+    return r;
+}
+__device__ __forceinline__ float &el(float4 &v, int e) { return (&v.x)[e]; }
+__device__ __forceinline__ float elc(const float4 &v, int e) { return (&v.x)[e]; }
+
+template <typename P, int TYV>
+__global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
+    const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
+    float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
+    const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt_mu0, int64_t nbx,
+    int64_t nby, int64_t nbz, int remap) {
+    const int lane = threadIdx.x;
+    const Tile tl = tile_of(nbx, nby, nbz, remap != 0);
+    const int64_t i0 = (tl.bx * 64 + lane) * VX;
+    const int64_t j = tl.by * TYV + threadIdx.y;
+    // the whole wave takes part in the shuffles; out-of-range lanes only idle
+    const bool row_ok = j < ny - 1;
+    const bool act = row_ok && i0 < nx;
+    const int64_t k0 = tl.bz * kchunk;
+    const int64_t k1 = min(k0 + kchunk, nz - 1);
+    if (!row_ok || k0 >= k1) return;   // uniform across the wave (same j, same k range)
+    const int64_t sy = nx, sz = nx * ny;
+    const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
+    const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
+    const float *zk = (const float *)cp.ik[2], *za = (const float *)cp.a[2], *zb = (const float *)cp.b[2];
+    P **ps = (P **)cp.psi;
+    const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
+    const int64_t ic = act ? i0 : 0;   // clamp addresses of idle lanes
+    int64_t jsy;
+    const int ry = pml_slab(j, ny, wy, &jsy);
+    const float4 idx4 = ld4(idcx + ic), xk4 = ld4(xk + ic);
+    const float idy_ = idcy[j], yk_ = yk[j], ya_ = ya[j], yb_ = yb[j];
+    const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
+    int64_t o = ic + sy * j + sz * k0;
+    float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o);
+    for (int64_t k = k0; k < k1; ++k, o += sz) {
+        const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
+        const float4 ez_c = ld4(Ez + o);
+        const float4 ez_y = ld4(Ez + o + sy), ex_y = ld4(Ex + o + sy);
+        float4 hx = ld4nt(Hx + o), hy = ld4nt(Hy + o), hz = ld4nt(Hz + o);
+        // x neighbours of element 3: element 0 of the next lane's group
+        float ez_nx = __shfl_down(ez_c.x, 1), ey_nx = __shfl_down(ey_c.x, 1);
+        if (lane == 63 && ic + VX < nx) {
+            ez_nx = Ez[o + VX];
+            ey_nx = Ey[o + VX];
+        }
+        const float idz_ = idcz[k], zk_ = zk[k];
+        int64_t jsz;
+        const int rz = pml_slab(k, nz, wz, &jsz);
+        const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            const int64_t i = ic + e;
+            if (!act || i >= nx - 1) continue;   // H at i = nx-1 keeps its value
+            const float ezc = elc(ez_c, e), eyc = elc(ey_c, e), exc = elc(ex_c, e);
+            const float ez_xp = e < VX - 1 ? elc(ez_c, e + 1) : ez_nx;
+            const float ey_xp = e < VX - 1 ? elc(ey_c, e + 1) : ey_nx;
+            const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+            const float dEz_dy = (elc(ez_y, e) - ezc) * idy_;
+            const float dEy_dz = (elc(ey_n, e) - eyc) * idz_;
+            const float dEx_dz = (elc(ex_n, e) - exc) * idz_;
+            const float dEz_dx = (ez_xp - ezc) * idx_;
+            const float dEy_dx = (ey_xp - eyc) * idx_;
+            const float dEx_dy = (elc(ex_y, e) - exc) * idy_;
+            int64_t jsx = 0;
+            const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
+            const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
+            const int64_t ox = jsx + wx * (j + ny * k);
+            const int64_t oy = i + nx * (jsy + wy * k);
+            const int64_t oz = i + nx * (j + ny * jsz);
+            const float pHxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
+            const float pHxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
+            el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz));
+            const float pHyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dEx_dz, cp.sc, cp.isc);
+            const float pHyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dEz_dx, cp.sc, cp.isc);
+            el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx));
+            const float pHzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dEy_dx, cp.sc, cp.isc);
+            const float pHzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dEx_dy, cp.sc, cp.isc);
+            el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy));
+        }
+        if (act) {
+            st4nt(Hx + o, hx);
+            st4nt(Hy + o, hy);
+            st4nt(Hz + o, hz);
+        }
+        ex_c = ex_n;
+        ey_c = ey_n;
+    }
+}
+
+template <typename P, int TYV>
+__global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
+    float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ Ez,
+    const float *__restrict__ Hx, const float *__restrict__ Hy, const float *__restrict__ Hz,
+    float *__restrict__ Dx, float *__restrict__ Dy, float *__restrict__ Dz,
+    const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
+    const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, int64_t nby,
+    int64_t nbz, int remap) {
+    const int lane = threadIdx.x;
+    const Tile tl = tile_of(nbx, nby, nbz, remap != 0);
+    const int64_t i0 = (tl.bx * 64 + lane) * VX;
+    const int64_t j = tl.by * TYV + threadIdx.y;
+    const bool row_ok = j >= 1 && j < ny;
+    const bool act = row_ok && i0 < nx;
+    const int64_t k0 = max(tl.bz * kchunk, (int64_t)1);
+    const int64_t k1 = min(tl.bz * kchunk + kchunk, nz);
+    if (!row_ok || k0 >= k1) return;
+    const int64_t sy = nx, sz = nx * ny;
+    const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
+    const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
+    const float *zk = (const float *)cp.ik[2], *za = (const float *)cp.a[2], *zb = (const float *)cp.b[2];
+    P **ps = (P **)cp.psi;
+    const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
+    const int64_t ic = act ? i0 : 0;
+    int64_t jsy;
+    const int ry = pml_slab(j, ny, wy, &jsy);
+    const float4 idx4 = ld4(iddx + ic), xk4 = ld4(xk + ic);
+    const float idy_ = iddy[j], yk_ = yk[j], ya_ = ya[j], yb_ = yb[j];
+    const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
+    int64_t o = ic + sy * j + sz * k0;
+    float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
+    for (int64_t k = k0; k < k1; ++k, o += sz) {
+        const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
+        const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
+        float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
+        float4 ex, ey, ez;   // fully overwritten except cell i = 0 (stored element-wise below)
+        const float4 iex4 = ld4nt(iex + o), iey4 = ld4nt(iey + o), iez4 = ld4nt(iez + o);
+        // x neighbours of element 0: element 3 of the previous lane's group
+        float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
+        if (lane == 0 && ic > 0) {
+            hz_px = Hz[o - 1];
+            hy_px = Hy[o - 1];
+        }
+        const float idz_ = iddz[k], zk_ = zk[k];
+        int64_t jsz;
+        const int rz = pml_slab(k, nz, wz, &jsz);
+        const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            const int64_t i = ic + e;
+            if (!act || i < 1) continue;   // E, D at i = 0 keep their values
+            const float hxc = elc(hx_c, e), hyc = elc(hy_c, e), hzc = elc(hz_c, e);
+            const float hz_xm = e > 0 ? elc(hz_c, e - 1) : hz_px;
+            const float hy_xm = e > 0 ? elc(hy_c, e - 1) : hy_px;
+            const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+            const float dHz_dy = (hzc - elc(hz_y, e)) * idy_;
+            const float dHy_dz = (hyc - elc(hy_p, e)) * idz_;
+            const float dHx_dz = (hxc - elc(hx_p, e)) * idz_;
+            const float dHz_dx = (hzc - hz_xm) * idx_;
+            const float dHy_dx = (hyc - hy_xm) * idx_;
+            const float dHx_dy = (hxc - elc(hx_y, e)) * idy_;
+            int64_t jsx = 0;
+            const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
+            const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
+            const int64_t ox = jsx + wx * (j + ny * k);
+            const int64_t oy = i + nx * (jsy + wy * k);
+            const int64_t oz = i + nx * (j + ny * jsz);
+            const float pDxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
+            const float pDxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
+            const float cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
+            const float pDyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dHx_dz, cp.sc, cp.isc);
+            const float pDyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dHz_dx, cp.sc, cp.isc);
+            const float cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
+            const float pDzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dHy_dx, cp.sc, cp.isc);
+            const float pDzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dHx_dy, cp.sc, cp.isc);
+            const float cz_term = (dHy_dx * xk_ + pDzx) - (dHx_dy * yk_ + pDzy);
+            const float dxn = el(dx, e) + dt * cx_term;
+            const float dyn = el(dy, e) + dt * cy_term;
+            const float dzn = el(dz, e) + dt * cz_term;
+            el(dx, e) = dxn;
+            el(dy, e) = dyn;
+            el(dz, e) = dzn;
+            el(ex, e) = dxn * elc(iex4, e);
+            el(ey, e) = dyn * elc(iey4, e);
+            el(ez, e) = dzn * elc(iez4, e);
+        }
+        if (act && ic > 0) {
+            st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
+            st4nt(Ex + o, ex); st4nt(Ey + o, ey); st4nt(Ez + o, ez);
+        } else if (act) {   // the group holding i = 0: that cell is not part of the E update
+            for (int e = 1; e < VX; ++e) {
+                Dx[o + e] = elc(dx, e); Dy[o + e] = elc(dy, e); Dz[o + e] = elc(dz, e);
+                Ex[o + e] = elc(ex, e); Ey[o + e] = elc(ey, e); Ez[o + e] = elc(ez, e);
+            }
+        }
+        hx_p = hx_c;
+        hy_p = hy_c;
+    }
+}
+
 // z-chunk length: enough workgroups to fill 256 CUs several times over.
 static int64_t pick_kchunk(int64_t nx, int64_t ny, int64_t nz) {
     const int64_t tiles = (int64_t)cdiv(nx, TX) * (int64_t)cdiv(ny, TY);
@@ -241,6 +470,63 @@ static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
         if (!c->inv_kappa[d] || !c->a[d] || !c->b[d]) return fail_arg("%s: missing CPML coefficients", who);
     }
     return MP_OK;
+}
+
+// Vector-form launch shape: 64 lanes x TYV rows per workgroup, z chunks of VKC planes.
+constexpr int TYV = 4;
+static int64_t vec_kchunk() {
+    static int64_t kc = [] {
+        const char *e = getenv("MP_YEE_KCHUNK");
+        const long v = e ? atol(e) : 2;
+        return (int64_t)(v > 0 ? v : 2);
+    }();
+    return kc;
+}
+// Measured on the 512x512x256 step the remap costs ~3% (the two x tiles of a row then sit
+// on different XCDs), so it is opt-in (MP_YEE_REMAP=1).
+static int xcd_remap() {
+    static const int on = getenv("MP_YEE_REMAP") != nullptr;
+    return on;
+}
+
+template <typename P>
+static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int64_t kc = vec_kchunk();
+    const int64_t nbx = cdiv(nx, 64 * VX), nby = cdiv(ny, TYV), nbz = cdiv(nz, kc);
+    dim3 block(64, TYV, 1);
+    const float dt_mu0 = (float)dt / (float)mu0;
+    KernelTimer t("yee_H");
+    yee_H_vec_kernel<P, TYV><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
+        (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+        (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, kc, dt_mu0, nbx, nby, nbz,
+        xcd_remap());
+    MP_LAUNCH_CHECK("yee_H_vec_kernel");
+    return MP_OK;
+}
+
+template <typename P>
+static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
+    const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    const int64_t kc = vec_kchunk();
+    const int64_t nbx = cdiv(nx, 64 * VX), nby = cdiv(ny, TYV), nbz = cdiv(nz, kc);
+    dim3 block(64, TYV, 1);
+    KernelTimer t("yee_E");
+    yee_E_vec_kernel<P, TYV><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
+        (float *)g->ex, (float *)g->ey, (float *)g->ez, (const float *)g->hx, (const float *)g->hy,
+        (const float *)g->hz, (float *)g->dx, (float *)g->dy, (float *)g->dz, (const float *)g->inv_eps_x,
+        (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, (const float *)g->inv_d_dual_x,
+        (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz,
+        kc, (float)dt, nbx, nby, nbz, xcd_remap());
+    MP_LAUNCH_CHECK("yee_E_vec_kernel");
+    return MP_OK;
+}
+
+// The vector form needs x rows of a multiple of 4 cells (16-byte aligned rows).
+static bool vec_ok(const mp_yee *g) {
+    static const bool off = getenv("MP_YEE_SCALAR") != nullptr;
+    return !off && g->nx % VX == 0;
 }
 
 template <typename S, typename C, typename P>
@@ -284,7 +570,11 @@ MP_API int mp_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0) {
     if (rc) return rc;
     if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:528
     const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
-    if (fd == MP_F32 && cd == MP_F32) {
+    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g)) {
+        if (pd == MP_F32) return launch_H_vec<float>(g, c, dt, mu0);
+        if (pd == MP_BF16) return launch_H_vec<bf16>(g, c, dt, mu0);
+        if (pd == MP_F16) return launch_H_vec<f16s>(g, c, dt, mu0);
+    } else if (fd == MP_F32 && cd == MP_F32) {
         if (pd == MP_F32) return launch_H<float, float, float>(g, c, dt, mu0);
         if (pd == MP_BF16) return launch_H<float, float, bf16>(g, c, dt, mu0);
         if (pd == MP_F16) return launch_H<float, float, f16s>(g, c, dt, mu0);
@@ -302,7 +592,11 @@ MP_API int mp_update_E_3d(const mp_yee *g, mp_cpml *c, double dt) {
     if (rc) return rc;
     if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:550
     const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
-    if (fd == MP_F32 && cd == MP_F32) {
+    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g)) {
+        if (pd == MP_F32) return launch_E_vec<float>(g, c, dt);
+        if (pd == MP_BF16) return launch_E_vec<bf16>(g, c, dt);
+        if (pd == MP_F16) return launch_E_vec<f16s>(g, c, dt);
+    } else if (fd == MP_F32 && cd == MP_F32) {
         if (pd == MP_F32) return launch_E<float, float, float>(g, c, dt);
         if (pd == MP_BF16) return launch_E<float, float, bf16>(g, c, dt);
         if (pd == MP_F16) return launch_E<float, float, f16s>(g, c, dt);
