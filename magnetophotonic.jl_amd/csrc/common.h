@@ -19,6 +19,8 @@ void set_error(const char *fmt, ...);
 int fail_arg(const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 hipStream_t stream();
+// Temporarily redirect launches to `s`; returns the previous stream.
+hipStream_t swap_stream(hipStream_t s);
 
 // per-kernel timing hooks (no-ops unless mp_profile_enable(1))
 struct KernelTimer {

@@ -6,6 +6,7 @@
 // src/fdtd/Source.jl:1-33 and the per-step device record! of ProbeReadout
 // (src/sim/ProbeReadout.jl:274-307).
 #include <cmath>
+#include <cstdlib>
 
 #include "common.h"
 
@@ -40,6 +41,38 @@ static int inject(mp_state *s) {
     }
 }
 
+// Side stream for the film integrator.  The 4TM + LLB update of step n is FP64-compute
+// bound while the next step's Yee updates are HBM bound; its only consumers are that
+// step's MO gyration (reads m) and absorbed-power accumulation (writes the window it
+// reads), so it runs on a second stream and the main stream waits on its event just
+// before those two kernels.  Both streams keep the default priority.
+struct Overlap {
+    hipStream_t side = nullptr;
+    hipEvent_t main_done = nullptr, film_done = nullptr;
+    bool pending = false;   // film_done recorded and not yet waited on
+};
+static Overlap g_ov;
+
+static int overlap_init() {
+    if (g_ov.side) return MP_OK;
+    MP_HIP(hipStreamCreateWithFlags(&g_ov.side, hipStreamNonBlocking));
+    MP_HIP(hipEventCreateWithFlags(&g_ov.main_done, hipEventDisableTiming));
+    MP_HIP(hipEventCreateWithFlags(&g_ov.film_done, hipEventDisableTiming));
+    return MP_OK;
+}
+
+static int overlap_join() {
+    if (!g_ov.pending) return MP_OK;
+    MP_HIP(hipStreamWaitEvent(stream(), g_ov.film_done, 0));
+    g_ov.pending = false;
+    return MP_OK;
+}
+
+static bool overlap_enabled() {
+    static const bool on = getenv("MP_NO_OVERLAP") == nullptr;
+    return on;
+}
+
 static int one_step(mp_state *s) {
     int rc;
     if ((rc = mp_update_H_3d(s->yee, s->cpml, s->dt, s->mu0))) return rc;
@@ -50,6 +83,7 @@ static int one_step(mp_state *s) {
         for (int c = 0; c < 3; ++c)
             if ((rc = mp_patch_E_dispersive(E[c], s->ade[c], s->yee->field_dtype, s->dt))) return rc;
     }
+    if ((rc = overlap_join())) return rc;   // the previous step's film update is complete
     if (s->mo && s->film) {
         if ((rc = mp_patch_E_mo_gyration(s->mo_ea, s->mo_eb, s->mo, s->mo_iea, s->mo_ieb,
                                          s->yee->field_dtype, s->film->m_TM_x, s->film->m_RE_x, s->Q_TM,
@@ -65,9 +99,21 @@ static int one_step(mp_state *s) {
         if (s->multiphysics_enabled && s->multiphysics_every > 0 && s->n % s->multiphysics_every == 0) {
             const int64_t window = s->n - s->last_mp_n;
             if (window > 0) {
-                if ((rc = mp_multiphysics_step(s->film, s->params, s->lut, s->dt * (double)window, s->subcycles,
-                                               s->brillouin_iters, 1)))
-                    return rc;
+                const bool ov = overlap_enabled();
+                if (ov) {
+                    if ((rc = overlap_init())) return rc;
+                    MP_HIP(hipEventRecord(g_ov.main_done, stream()));
+                    MP_HIP(hipStreamWaitEvent(g_ov.side, g_ov.main_done, 0));
+                }
+                hipStream_t prev = ov ? swap_stream(g_ov.side) : stream();
+                rc = mp_multiphysics_step(s->film, s->params, s->lut, s->dt * (double)window, s->subcycles,
+                                          s->brillouin_iters, 1);
+                if (ov) swap_stream(prev);
+                if (rc) return rc;
+                if (ov) {
+                    MP_HIP(hipEventRecord(g_ov.film_done, g_ov.side));
+                    g_ov.pending = true;
+                }
                 s->last_mp_n = s->n;
             }
         }
@@ -86,7 +132,11 @@ MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
     if (s->ade[0] && (!s->ade[1] || !s->ade[2])) return fail_arg("mp_run_steps: ADE needs all three lists");
     for (int64_t k = 0; k < nsteps; ++k) {
         int rc = one_step(s);
-        if (rc) return rc;
+        if (rc) {
+            overlap_join();
+            return rc;
+        }
     }
-    return MP_OK;
+    // everything issued here is ordered before later work on the caller's stream
+    return overlap_join();
 }

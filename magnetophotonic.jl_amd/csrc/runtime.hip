@@ -48,6 +48,12 @@ int check_hip(hipError_t e, const char *what) {
 
 hipStream_t stream() { return g_stream; }
 
+hipStream_t swap_stream(hipStream_t s) {
+    hipStream_t old = g_stream;
+    g_stream = s;
+    return old;
+}
+
 static hipEvent_t pool_event() {
     if (!g_event_pool.empty()) {
         hipEvent_t e = g_event_pool.back();

@@ -233,3 +233,40 @@ def test_em_boxes_bitwise_f32(shape):
         assert np.any(a != 0) or name.startswith("d")
         assert np.array_equal(a, b), (shape, name, np.abs(a - b).max())
     gpu.free()
+
+
+def test_coupled_pump_batched_run_overlap():
+    """Many steps per mp_run_steps call: the film update of step n overlaps step n+1 on a
+    side stream; results must match the step-by-step oracle, monitors included."""
+    import driver
+    from mphip import configs
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+    spec, meta = configs.coupled_pump(nx=44, ny=40, nz=64, steps=240, fluence_mj_cm2=14.0, npml=6,
+                                       src_z=40e-9)
+    print("case coupled_pump batched 44x40x64", flush=True)
+    gpu = FDTDState(spec)
+    mon = DFTPlanes(gpu, meta["planes"], meta["omega_bins"])
+    gpu.attach_dft(mon)
+    ref = driver.OracleState(spec)
+    ref_mon = OracleDFT(ref, meta["planes"], meta["omega_bins"])
+    gpu.run(120)
+    gpu.run(120)
+    for _ in range(240):
+        ref.step()
+        ref_mon.record(ref)
+    gpu.synchronize()
+    assert gpu.n == 240 and abs(gpu.t - ref.t) <= 1e-12 * ref.t
+    for name in FIELDS:
+        assert rel_l2(gpu.field(name), ref.field(name)) < 1e-5, name
+    fa, fb = gpu.film_state(), ref.film_state()
+    assert fb["Te"].max() > 300.5
+    for k in ("Te", "Tl", "Ts_TM", "Ts_RE"):
+        assert rel_l2(fa[k], fb[k]) < 1e-6, k
+    for k in ("m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z"):
+        assert np.abs(fa[k].astype(np.float64) - fb[k]).max() < 1e-4, k
+    for p in range(len(meta["planes"])):
+        g, r = mon.download(p), ref_mon.download(p)
+        for k in ("ea", "eb", "ha", "hb", "energy"):
+            assert rel_l2(g[k], r[k]) < 1e-5, (p, k)
+    gpu.free()
