@@ -44,7 +44,7 @@ struct KernelTimer {
 
 // ---- storage types ----------------------------------------------------------------------
 struct bf16 { uint16_t v; };
-struct f16s { __half v; };
+struct f16s { uint16_t v; };  // IEEE binary16 bit pattern
 
 // Load a stored value as the compute type; store with round-to-nearest-even.
 template <typename C, typename S> __device__ __forceinline__ C ld(const S &x) { return (C)x; }
@@ -59,8 +59,28 @@ template <> __device__ __forceinline__ bf16 st<bf16, float>(float x) {
     u += 0x7fffu + ((u >> 16) & 1u);
     return bf16{(uint16_t)(u >> 16)};
 }
-template <> __device__ __forceinline__ float ld<float, f16s>(const f16s &x) { return __half2float(x.v); }
-template <> __device__ __forceinline__ f16s st<f16s, float>(float x) { return f16s{__float2half(x)}; }
+// binary16 in integer arithmetic: round to nearest even, subnormals kept, |x| >= 65520 to
+// infinity, NaN quieted, and the sign of zero kept (the hardware conversion maps -0 to +0).
+template <> __device__ __forceinline__ float ld<float, f16s>(const f16s &x) {
+    const uint32_t sign = (uint32_t)(x.v & 0x8000u) << 16;
+    const uint32_t e = (x.v >> 10) & 0x1fu, m = x.v & 0x3ffu;
+    if (e == 0) return __uint_as_float(__float_as_uint((float)m * 5.9604644775390625e-8f) | sign);
+    if (e == 31) return __uint_as_float(sign | 0x7f800000u | (m << 13));
+    return __uint_as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+template <> __device__ __forceinline__ f16s st<f16s, float>(float x) {
+    uint32_t u = __float_as_uint(x);
+    const uint16_t sign = (uint16_t)((u >> 16) & 0x8000u);
+    u &= 0x7fffffffu;
+    if (u > 0x7f800000u) return f16s{(uint16_t)(sign | 0x7e00u | ((u >> 13) & 0x3ffu))};
+    if (u >= 0x477ff000u) return f16s{(uint16_t)(sign | 0x7c00u)};
+    if (u >= 0x38800000u) {
+        u += 0xfffu + ((u >> 13) & 1u);
+        return f16s{(uint16_t)(sign | ((u - 0x38000000u) >> 13))};
+    }
+    const float q = __uint_as_float(u) * 16777216.0f;  // |x| / 2^-24, exact
+    return f16s{(uint16_t)(sign | (uint16_t)__builtin_rintf(q))};
+}
 
 // ---- Julia float semantics ----------------------------------------------------------------
 // max/min propagate NaN and order -0.0 < 0.0 (Base.max / Base.min for IEEE floats);

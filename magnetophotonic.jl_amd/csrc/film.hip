@@ -29,7 +29,8 @@ template <typename S, typename C, typename T>
 __global__ __launch_bounds__(256) void ade_patch_kernel(
     S *__restrict__ E, T *__restrict__ P, T *__restrict__ J, C *__restrict__ E_old,
     const int32_t *__restrict__ idxs, const C *__restrict__ fill, const C *__restrict__ inv_eps,
-    const C *__restrict__ poles, int np, int64_t N, C inv_alpha, float sc, float isc) {
+    const C *__restrict__ poles, int np, int64_t N, C inv_alpha, float sc, float isc, float jsc,
+    float ijsc) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     const int64_t idx = idxs[n];
@@ -42,7 +43,7 @@ __global__ __launch_bounds__(256) void ade_patch_kernel(
     for (int p = 0; p < MAXP; ++p) {
         if (p >= np) break;
         Pold[p] = StateIO<C, T>::load(P, n + p * N, isc);
-        Jold[p] = StateIO<C, T>::load(J, n + p * N, isc);
+        Jold[p] = StateIO<C, T>::load(J, n + p * N, ijsc);
         const C beta = poles[3 * p + 2] * f;
         const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
         sum_psi += psi;
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(256) void ade_patch_kernel(
             const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
             const C Pnew = psi + beta * Efinal;
             StateIO<C, T>::store(P, n + p * N, Pnew, sc);
-            StateIO<C, T>::store(J, n + p * N, (Pnew - Pold[p]) * inv_alpha - Jold[p], sc);
+            StateIO<C, T>::store(J, n + p * N, (Pnew - Pold[p]) * inv_alpha - Jold[p], jsc);
         }
     }
 }
@@ -158,16 +159,24 @@ __global__ __launch_bounds__(256) void pabs_kernel(
     if (pabs > peak[n]) peak[n] = pabs;
 }
 
+// Reduced-precision P/J storage (BF16, FP16): P is stored as P*scale and J in P units,
+// J*(dt/2)*scale, so one scale fits both (|J| ~ w|P| is ~1e15 larger than |P|).
+static inline bool reduced(int32_t d) { return d == MP_BF16 || d == MP_F16; }
+static inline double state_scale(const mp_ade *s) {
+    return reduced(s->state_dtype) && s->state_scale > 0 ? s->state_scale : 1.0;
+}
+
 template <typename S, typename C, typename T>
 static int launch_ade(void *E, mp_ade *s, double dt) {
     const int64_t N = s->n;
-    const double sc = (s->state_dtype == MP_BF16 || s->state_dtype == MP_F16) && s->state_scale > 0 ? s->state_scale : 1.0;
+    const double sc = state_scale(s);
     const C inv_alpha = (C)2 / (C)dt;
+    const double jsc = reduced(s->state_dtype) ? sc * dt / 2.0 : 1.0;
     KernelTimer t("ade_patch");
     ade_patch_kernel<S, C, T><<<cdiv(N, 256), 256, 0, stream()>>>(
         (S *)E, (T *)s->P, (T *)s->J, (C *)s->E_old, s->idx, (const C *)s->fill,
         (const C *)s->inv_eps, (const C *)s->poles, s->npoles, N, inv_alpha, (float)sc,
-        (float)(1.0 / sc));
+        (float)(1.0 / sc), (float)jsc, (float)(1.0 / jsc));
     MP_LAUNCH_CHECK("ade_patch_kernel");
     return MP_OK;
 }
@@ -178,7 +187,8 @@ static int launch_pabs(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_a
     const int64_t N = f->n;
     KernelTimer t("pabs");
     if (has_ade_work) {
-        const double sc = (ax->state_dtype == MP_BF16 || ax->state_dtype == MP_F16) && ax->state_scale > 0 ? ax->state_scale : 1.0;
+        // J load factor: 1 for CT storage, 1/(scale*dt/2) for reduced storage
+        const double sc = reduced(ax->state_dtype) ? state_scale(ax) * dt / 2.0 : 1.0;
         pabs_kernel<S, C, T, true><<<cdiv(N, 256), 256, 0, stream()>>>(
             f->U_abs, f->window_sum, f->peak, (const S *)g->ex, (const S *)g->ey, (const S *)g->ez,
             f->cells, f->pos_x, f->pos_y, f->pos_z, (const T *)ax->J, (const T *)ay->J,

@@ -101,6 +101,13 @@ def small_coupled(dtype: str = "f32", steps: int = 2000, amplitude: float = 2.0e
     return spec, meta
 
 
+# Storage scales of the reduced-precision form.  BF16 keeps FP32's exponent range, so it
+# needs none.  FP16 (max 65504) needs power-of-two scales chosen from the magnitudes of
+# the coupled pump at 60 mJ/cm^2: |psi_H| <= 2.4e19, |psi_D| <= 1.6e17, |P| <= 0.61 and
+# |J dt/2| <= 1.3e-3 (J is stored in P units), leaving a factor ~2 of headroom.
+REDUCED_SCALES = {"bf16": (1.0, 1.0), "f16": (2.0 ** -52, 2.0 ** 13)}
+
+
 def film_graded_axis(n_cells: int, d_fine: float = 2e-9, n_fine: int = 10, d_max: float = 20e-9,
                      stretch: float = 1.25) -> G.Axis1D:
     """A graded axis of exactly n_cells cells centred on 0: n_fine cells of d_fine in the
@@ -150,11 +157,14 @@ def coupled_pump(nx: int = 512, ny: int = 512, nz: int = 256, dtype: str = "f32"
                                   omega=2 * math.pi * G.C0 / LAMBDA_PUMP)
     src = sources.PlaneSource(pulse, "Ex", axis="z", position=float(zc[k_src]))
     gd = models.ferrimagnet_parameters()
+    psi_scale = REDUCED_SCALES.get(psi_precision, (1.0, 1.0))[0]
+    ade_scale = REDUCED_SCALES.get(ade_state_precision, (1.0, 1.0))[1]
     spec = build_state_spec(grid, geo, dt, compute_precision=dtype, field_precision=dtype, source=src,
                             n_pml=npml, cpml_kwargs=CPML_KW, model=gd, enable_dispersion=True,
                             enable_magneto_optic=enable_mo, diag_poles=film_poles(dt),
                             absorption_model="ade_work", multiphysics_every=1, subcycles=subcycles,
-                            psi_precision=psi_precision, ade_state_precision=ade_state_precision,
+                            psi_precision=psi_precision, psi_scale=psi_scale,
+                            ade_state_precision=ade_state_precision, ade_state_scale=ade_scale,
                             mo_pair=("x", "y"))
     CT = spec.compute_dtype
     for k, v in (("m_TM_x", 0.9), ("m_RE_x", -0.9), ("m_TM_y", 0.0), ("m_RE_y", 0.0),

@@ -449,10 +449,19 @@ class FDTDState:
     def set_film(self, key: str, values: np.ndarray):
         self.film_bufs[key].upload(np.asarray(values).astype(self.spec.compute_dtype))
 
+    def psi(self, which: str, s: int) -> np.ndarray:
+        """Raw storage of CPML slab s (0..11, order xy_lo .. zy_hi) of the H ('h') or
+        D ('d') update; BF16 / binary16 kinds come back as their uint16 bit patterns."""
+        buf = (self.psi_h if which == "h" else self.psi_d)[s]
+        dt = {"f32": np.float32, "f64": np.float64, "bf16": np.uint16, "f16": np.uint16}[self.spec.psi_dtype]
+        return buf.download(dt) if buf.ptr else np.zeros(0, dt)
+
     def ade_state(self, comp: int) -> dict:
+        """P, J and E_old of ADE list comp; reduced kinds come back as uint16 bit patterns
+        of P*scale and J*(dt/2)*scale."""
         sd = self.spec.ade_state_dtype
         b = self.ade_bufs[comp]
-        dt = {"f32": np.float32, "f64": np.float64, "bf16": np.uint16, "f16": np.float16}[sd]
+        dt = {"f32": np.float32, "f64": np.float64, "bf16": np.uint16, "f16": np.uint16}[sd]
         return dict(P=b["P"].download(dt), J=b["J"].download(dt), E_old=b["E_old"].download(self.spec.compute_dtype))
 
     def configure_probe_mode(self, lambda0: float = 532e-9, reset_ade: bool = True,

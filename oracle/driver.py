@@ -46,8 +46,15 @@ def lib(abi):
         h.orc_brillouin_fn.argtypes = [C.c_double, C.c_double]
         h.orc_lut_lookup.restype = C.c_double
         h.orc_lut_lookup.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32]
+        for name in ("orc_store_reduced", "orc_load_reduced"):
+            getattr(h, name).restype = None
+            getattr(h, name).argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_float]
         _lib = h
     return _lib
+
+
+# storage kind codes of include/mphip.h
+_KIND = {"f32": 0, "f64": 1, "bf16": 2, "f16": 3}
 
 
 def _ptr(a):
@@ -93,9 +100,9 @@ class OracleState:
         cs = spec.cpml
         cp = abi.Cpml()
         cp.npml_x, cp.npml_y, cp.npml_z = cs.widths
-        psi_np = {"f32": np.float32, "f64": np.float64}[spec.psi_dtype]
-        cp.psi_dtype = abi.MP_F32 if psi_np == np.float32 else abi.MP_F64
-        cp.psi_scale = 1.0
+        psi_np = {"f32": np.float32, "f64": np.float64, "bf16": np.uint16, "f16": np.uint16}[spec.psi_dtype]
+        cp.psi_dtype = _KIND[spec.psi_dtype]
+        cp.psi_scale = spec.psi_scale
         self.coef = []
         for d, ax in enumerate(cs.axes):
             arrs = (ax.inv_kappa.copy(), ax.a.copy(), ax.b.copy())
@@ -123,10 +130,13 @@ class OracleState:
             self.ade, self.ade_arr = [], []
             for lst in spec.ade:
                 n = len(lst.idx)
-                arr = dict(P=np.zeros(n * np_, CT), J=np.zeros(n * np_, CT), E_old=np.zeros(n, CT),
+                sd = spec.ade_state_dtype
+                SD = np.uint16 if sd in ("bf16", "f16") else CT
+                arr = dict(P=np.zeros(n * np_, SD), J=np.zeros(n * np_, SD), E_old=np.zeros(n, CT),
                            idx=lst.idx.copy(), fill=lst.fill.copy(), inv=lst.inv.copy())
                 a = abi.Ade()
-                a.n, a.npoles, a.dtype, a.state_dtype, a.state_scale = n, np_, self.ccode, self.ccode, 1.0
+                a.n, a.npoles, a.dtype = n, np_, self.ccode
+                a.state_dtype, a.state_scale = _KIND[sd], spec.ade_state_scale
                 a.P, a.J, a.E_old = _ptr(arr["P"]), _ptr(arr["J"]), _ptr(arr["E_old"])
                 a.idx, a.fill, a.inv_eps = _ptr(arr["idx"]), _ptr(arr["fill"]), _ptr(arr["inv"])
                 a.poles = _ptr(self.poles)

@@ -30,6 +30,10 @@
 #define ORC_EXPORT __attribute__((visibility("default")))
 
 #include "orc_physics.h"
+#include "orc_half.h"
+
+#define PSI_LD(p, o) ((CT)(p)[o])
+#define PSI_ST(p, o, v) ((p)[o] = (PT)(v))
 
 /* ---- Yee instantiations: (ST, CT, PT) ------------------------------------------------- */
 #define ST float
@@ -79,6 +83,32 @@
 #undef PT
 #undef SFX
 #undef CFMA
+#undef PSI_LD
+#undef PSI_ST
+
+/* FP32 fields and arithmetic with psi stored as BF16 / binary16 holding psi*psc */
+#define ST float
+#define CT float
+#define PT uint16_t
+#define CFMA fmaf
+#define SFX(n) n##_ffb
+#define PSI_LD(p, o) (orc_bf16_to_f32((p)[o]) * pisc)
+#define PSI_ST(p, o, v) ((p)[o] = orc_f32_to_bf16((v) * psc))
+#include "orc_yee.h"
+#undef SFX
+#undef PSI_LD
+#undef PSI_ST
+#define SFX(n) n##_ffh
+#define PSI_LD(p, o) (orc_f16_to_f32((p)[o]) * pisc)
+#define PSI_ST(p, o, v) ((p)[o] = orc_f32_to_f16((v) * psc))
+#include "orc_yee.h"
+#undef SFX
+#undef PSI_LD
+#undef PSI_ST
+#undef ST
+#undef CT
+#undef PT
+#undef CFMA
 
 /* ---- film / source instantiations: (ST, CT) -------------------------------------------- */
 #define ST float
@@ -119,6 +149,15 @@ static int orc_fail(const char *msg) {
 ORC_EXPORT const char *orc_last_error(void) { return orc_err; }
 ORC_EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
 
+/* element-wise storage conversions of the reduced kinds, exported for their own tests:
+ * out[i] = bits of x[i]*sc, and back, y[i] = value(h[i])*isc */
+ORC_EXPORT void orc_store_reduced(int32_t kind, const float *x, uint16_t *out, int64_t n, float sc) {
+    for (int64_t i = 0; i < n; ++i) orc_sst(out, i, kind, x[i], sc);
+}
+ORC_EXPORT void orc_load_reduced(int32_t kind, const uint16_t *h, float *y, int64_t n, float isc) {
+    for (int64_t i = 0; i < n; ++i) y[i] = (float)orc_sld(h, i, kind, isc);
+}
+
 /* storage/compute pair code */
 static int pair_code(int32_t st, int32_t ct) {
     if (st == MP_F32 && ct == MP_F32) return 0;
@@ -127,17 +166,31 @@ static int pair_code(int32_t st, int32_t ct) {
     return -1;
 }
 
+/* storage scale of reduced-precision psi: psi is kept as psi*sc */
+static void psi_scales(const mp_cpml *c, float *sc, float *isc) {
+    double s = (c->psi_dtype == MP_BF16 || c->psi_dtype == MP_F16) ? c->psi_scale : 1.0;
+    if (!(s > 0)) s = 1.0;
+    *sc = (float)s;
+    *isc = (float)(1.0 / s);
+}
+
 ORC_EXPORT int orc_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0) {
     if (g->nx < 2 || g->ny < 2 || g->nz < 2) return 0;
     int pc = pair_code(g->field_dtype, g->compute_dtype);
+    float sc, isc;
+    psi_scales(c, &sc, &isc);
     if (pc == 0 && c->psi_dtype == MP_F32) {
-        orc_yee_H_fff(g, c, (float)dt / (float)mu0);
+        orc_yee_H_fff(g, c, (float)dt / (float)mu0, sc, isc);
+    } else if (pc == 0 && c->psi_dtype == MP_BF16) {
+        orc_yee_H_ffb(g, c, (float)dt / (float)mu0, sc, isc);
+    } else if (pc == 0 && c->psi_dtype == MP_F16) {
+        orc_yee_H_ffh(g, c, (float)dt / (float)mu0, sc, isc);
     } else if (pc == 1 && c->psi_dtype == MP_F64) {
-        orc_yee_H_ddd(g, c, dt / mu0);
+        orc_yee_H_ddd(g, c, dt / mu0, sc, isc);
     } else if (pc == 2 && c->psi_dtype == MP_F32) {
-        orc_yee_H_fdf(g, c, dt / mu0);
+        orc_yee_H_fdf(g, c, dt / mu0, sc, isc);
     } else if (pc == 1 && c->psi_dtype == MP_F32) {
-        orc_yee_H_ddf(g, c, dt / mu0);
+        orc_yee_H_ddf(g, c, dt / mu0, sc, isc);
     } else {
         return orc_fail("orc_update_H_3d: unsupported dtype combination");
     }
@@ -147,14 +200,20 @@ ORC_EXPORT int orc_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu
 ORC_EXPORT int orc_update_E_3d(const mp_yee *g, mp_cpml *c, double dt) {
     if (g->nx < 2 || g->ny < 2 || g->nz < 2) return 0;
     int pc = pair_code(g->field_dtype, g->compute_dtype);
+    float sc, isc;
+    psi_scales(c, &sc, &isc);
     if (pc == 0 && c->psi_dtype == MP_F32) {
-        orc_yee_E_fff(g, c, (float)dt);
+        orc_yee_E_fff(g, c, (float)dt, sc, isc);
+    } else if (pc == 0 && c->psi_dtype == MP_BF16) {
+        orc_yee_E_ffb(g, c, (float)dt, sc, isc);
+    } else if (pc == 0 && c->psi_dtype == MP_F16) {
+        orc_yee_E_ffh(g, c, (float)dt, sc, isc);
     } else if (pc == 1 && c->psi_dtype == MP_F64) {
-        orc_yee_E_ddd(g, c, dt);
+        orc_yee_E_ddd(g, c, dt, sc, isc);
     } else if (pc == 2 && c->psi_dtype == MP_F32) {
-        orc_yee_E_fdf(g, c, dt);
+        orc_yee_E_fdf(g, c, dt, sc, isc);
     } else if (pc == 1 && c->psi_dtype == MP_F32) {
-        orc_yee_E_ddf(g, c, dt);
+        orc_yee_E_ddf(g, c, dt, sc, isc);
     } else {
         return orc_fail("orc_update_E_3d: unsupported dtype combination");
     }
@@ -235,12 +294,34 @@ ORC_EXPORT int orc_inject_plane_3d(const mp_yee *g, int32_t component, int32_t a
     return 0;
 }
 
+static int reduced_kind(int32_t k) { return k == MP_BF16 || k == MP_F16; }
+
+/* P/J storage of an ADE list: kind, P scale and J scale.  Reduced kinds keep P*scale and J
+ * in P units, J*(dt/2)*scale; CT storage uses no scale. */
+static int ade_storage(const mp_ade *s, double dt, float *sc, float *isc, float *jsc, float *ijsc) {
+    if (!reduced_kind(s->state_dtype)) {
+        *sc = *isc = *jsc = *ijsc = 1.0f;
+        return s->dtype;
+    }
+    const double p = s->state_scale > 0 ? s->state_scale : 1.0;
+    const double j = p * dt / 2.0;
+    *sc = (float)p;
+    *isc = (float)(1.0 / p);
+    *jsc = (float)j;
+    *ijsc = (float)(1.0 / j);
+    return s->state_dtype;
+}
+
 ORC_EXPORT int orc_patch_E_dispersive(void *E, mp_ade *s, int32_t field_dtype, double dt) {
     if (s->n <= 0 || s->npoles <= 0) return 0;
-    switch (pair_code(field_dtype, s->dtype)) {
-    case 0: orc_ade_patch_ff((float *)E, s, 2.0f / (float)dt); break;
-    case 1: orc_ade_patch_dd((double *)E, s, 2.0 / dt); break;
-    case 2: orc_ade_patch_fd((float *)E, s, 2.0 / dt); break;
+    float sc, isc, jsc, ijsc;
+    const int sk = ade_storage(s, dt, &sc, &isc, &jsc, &ijsc);
+    const int pc = pair_code(field_dtype, s->dtype);
+    if (reduced_kind(sk) && pc != 0) return orc_fail("orc_patch_E_dispersive: reduced state needs FP32");
+    switch (pc) {
+    case 0: orc_ade_patch_ff((float *)E, s, 2.0f / (float)dt, sk, sc, isc, jsc, ijsc); break;
+    case 1: orc_ade_patch_dd((double *)E, s, 2.0 / dt, sk, sc, isc, jsc, ijsc); break;
+    case 2: orc_ade_patch_fd((float *)E, s, 2.0 / dt, sk, sc, isc, jsc, ijsc); break;
     default: return orc_fail("orc_patch_E_dispersive: unsupported dtypes");
     }
     return 0;
@@ -274,10 +355,12 @@ ORC_EXPORT int orc_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax
                                    double dt) {
     if (f->n <= 0) return 0;
     int ct = has_ade_work ? ax->dtype : f->dtype;
+    float sc = 1.0f, isc = 1.0f, jsc = 1.0f, ijsc = 1.0f;
+    const int sk = has_ade_work ? ade_storage(ax, dt, &sc, &isc, &jsc, &ijsc) : ct;
     switch (pair_code(g->field_dtype, ct)) {
-    case 0: orc_pabs_ff(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
-    case 1: orc_pabs_dd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
-    case 2: orc_pabs_fd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt); break;
+    case 0: orc_pabs_ff(f, g, ax, ay, az, has_ade_work, pabs_factor, dt, sk, ijsc); break;
+    case 1: orc_pabs_dd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt, sk, ijsc); break;
+    case 2: orc_pabs_fd(f, g, ax, ay, az, has_ade_work, pabs_factor, dt, sk, ijsc); break;
     default: return orc_fail("orc_pabs_accumulate: unsupported dtypes");
     }
     return 0;

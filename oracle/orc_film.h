@@ -52,10 +52,14 @@ static void SFX2(orc_inject_mode_x)(const mp_yee *g, int code, int64_t i, const 
 }
 
 /* ade_patch_kernel! (Kernels.jl:194-231) over the whole list. */
-static void SFX2(orc_ade_patch)(ST *E_arr, mp_ade *s, CT inv_alpha) {
+/* P/J storage kind sk: CT, or BF16/binary16 holding P*sc and J*jsc (read back with
+ * isc = 1/sc, ijsc = 1/jsc, FP32 products). */
+static void SFX2(orc_ade_patch)(ST *E_arr, mp_ade *s, CT inv_alpha, int sk, float sc, float isc,
+                                float jsc, float ijsc) {
     const int64_t N = s->n;
     const int np = s->npoles;
-    CT *P = (CT *)s->P, *J = (CT *)s->J, *E_old = (CT *)s->E_old;
+    void *P = s->P, *J = s->J;
+    CT *E_old = (CT *)s->E_old;
     const CT *fill = (const CT *)s->fill, *inv_eps = (const CT *)s->inv_eps;
     const CT *poles = (const CT *)s->poles;
 #pragma omp parallel for schedule(static)
@@ -68,7 +72,8 @@ static void SFX2(orc_ade_patch)(ST *E_arr, mp_ade *s, CT inv_alpha) {
         for (int p = 0; p < np; ++p) {
             CT C1 = poles[3 * p], C2 = poles[3 * p + 1], C3 = poles[3 * p + 2];
             CT beta = C3 * f;
-            CT psi = CFMA(C1, P[n + p * N], CFMA(C2, J[n + p * N], beta * Eprev));
+            CT psi = CFMA(C1, (CT)orc_sld(P, n + p * N, sk, isc),
+                          CFMA(C2, (CT)orc_sld(J, n + p * N, sk, ijsc), beta * Eprev));
             sum_psi += psi;
             sum_beta += beta;
         }
@@ -81,12 +86,12 @@ static void SFX2(orc_ade_patch)(ST *E_arr, mp_ade *s, CT inv_alpha) {
             for (int p = 0; p < np; ++p) {
                 CT C1 = poles[3 * p], C2 = poles[3 * p + 1], C3 = poles[3 * p + 2];
                 CT beta = C3 * f;
-                CT Pold = P[n + p * N];
-                CT Jold = J[n + p * N];
+                CT Pold = (CT)orc_sld(P, n + p * N, sk, isc);
+                CT Jold = (CT)orc_sld(J, n + p * N, sk, ijsc);
                 CT psi = CFMA(C1, Pold, CFMA(C2, Jold, beta * Eprev));
                 CT Pnew = psi + beta * Efinal;
-                P[n + p * N] = Pnew;
-                J[n + p * N] = (Pnew - Pold) * inv_alpha - Jold;
+                orc_sst(P, n + p * N, sk, Pnew, sc);
+                orc_sst(J, n + p * N, sk, (Pnew - Pold) * inv_alpha - Jold, jsc);
             }
         }
     }
@@ -161,7 +166,8 @@ static void SFX2(orc_mo_gyration)(ST *Ea, ST *Eb, mp_mo *s, const ST *iea, const
 
 /* pabs_accumulate_kernel! (Kernels.jl:313-355); FP64 throughout. */
 static void SFX2(orc_pabs)(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
-                           const mp_ade *az, int has_ade_work, double pabs_factor, double dt) {
+                           const mp_ade *az, int has_ade_work, double pabs_factor, double dt,
+                           int sk, float ijsc) {
     const ST *Ex = (const ST *)g->ex, *Ey = (const ST *)g->ey, *Ez = (const ST *)g->ez;
     const int64_t N = f->n;
 #pragma omp parallel for schedule(static)
@@ -174,9 +180,9 @@ static void SFX2(orc_pabs)(mp_film *f, const mp_yee *g, const mp_ade *ax, const 
             const int32_t px = f->pos_x[n], py = f->pos_y[n], pz = f->pos_z[n];
             const int np = ax->npoles;
             for (int p = 0; p < np; ++p) {
-                if (px >= 0) jx += (double)((const CT *)ax->J)[px + p * ax->n];
-                if (py >= 0) jy += (double)((const CT *)ay->J)[py + p * ay->n];
-                if (pz >= 0) jz += (double)((const CT *)az->J)[pz + p * az->n];
+                if (px >= 0) jx += orc_sld(ax->J, px + p * ax->n, sk, ijsc);
+                if (py >= 0) jy += orc_sld(ay->J, py + p * ay->n, sk, ijsc);
+                if (pz >= 0) jz += orc_sld(az->J, pz + p * az->n, sk, ijsc);
             }
             pabs = jl_max(ex * jx + ey * jy + ez * jz, 0.0);
         } else {

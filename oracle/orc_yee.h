@@ -7,7 +7,8 @@
  * Julia's muladd is written as fma; every other expression keeps Julia's evaluation
  * order and is compiled with -ffp-contract=off so no other contraction happens.
  *
- * Required macros: ST, CT, PT, SFX(name), CFMA (fma for CT).
+ * Required macros: ST, CT, PT, SFX(name), CFMA (fma for CT), PSI_LD(ptr, off) and
+ * PSI_ST(ptr, off, v) (psi storage; the reduced kinds use the scales psc/pisc in scope).
  */
 
 /* _pml_slab (Kernels.jl:9-14), 0-based: returns region 0 (none), 1 (lo) or 2 (hi) and the
@@ -23,21 +24,23 @@ static inline int SFX(pml_slab)(int64_t idx, int64_t N, int64_t npml, int64_t *j
 /* _cpml_term_{x,y,z}! (Kernels.jl:16-56): psi <- b*psi + a*deriv (muladd), only inside
  * a slab.  `off` is the linear index into the slab given the slab coordinate. */
 static inline CT SFX(cpml_term)(PT *lo, PT *hi, int region, int64_t off, const CT *b,
-                                const CT *a, int64_t idx, CT deriv) {
+                                const CT *a, int64_t idx, CT deriv, float psc, float pisc) {
+    (void)psc;
+    (void)pisc;
     if (region == 1) {
-        CT p = CFMA(b[idx], (CT)lo[off], a[idx] * deriv);
-        lo[off] = (PT)p;
+        CT p = CFMA(b[idx], PSI_LD(lo, off), a[idx] * deriv);
+        PSI_ST(lo, off, p);
         return p;
     } else if (region == 2) {
-        CT p = CFMA(b[idx], (CT)hi[off], a[idx] * deriv);
-        hi[off] = (PT)p;
+        CT p = CFMA(b[idx], PSI_LD(hi, off), a[idx] * deriv);
+        PSI_ST(hi, off, p);
         return p;
     }
     return (CT)0;
 }
 
 /* yee_H_3d_kernel! (Kernels.jl:60-101) over 0 <= i < nx-1, 0 <= j < ny-1, 0 <= k < nz-1. */
-static void SFX(orc_yee_H)(const mp_yee *g, mp_cpml *c, CT dt_mu0) {
+static void SFX(orc_yee_H)(const mp_yee *g, mp_cpml *c, CT dt_mu0, float psc, float pisc) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     const int64_t sy = nx, sz = nx * ny;
     const ST *Ex = (const ST *)g->ex, *Ey = (const ST *)g->ey, *Ez = (const ST *)g->ez;
@@ -68,16 +71,16 @@ static void SFX(orc_yee_H)(const mp_yee *g, mp_cpml *c, CT dt_mu0) {
                 const int64_t oy = i + nx * (jsy + wy * k);   /* y-slab (nx, w, nz) */
                 const int64_t oz = i + nx * (j + ny * jsz);   /* z-slab (nx, ny, w) */
 
-                CT pHxy = SFX(cpml_term)(ps[0], ps[1], ry, oy, yb, ya, j, dEz_dy);
-                CT pHxz = SFX(cpml_term)(ps[2], ps[3], rz, oz, zb, za, k, dEy_dz);
+                CT pHxy = SFX(cpml_term)(ps[0], ps[1], ry, oy, yb, ya, j, dEz_dy, psc, pisc);
+                CT pHxz = SFX(cpml_term)(ps[2], ps[3], rz, oz, zb, za, k, dEy_dz, psc, pisc);
                 Hx[o] = (ST)((CT)Hx[o] - dt_mu0 * ((dEz_dy * yk[j] + pHxy) - (dEy_dz * zk[k] + pHxz)));
 
-                CT pHyz = SFX(cpml_term)(ps[4], ps[5], rz, oz, zb, za, k, dEx_dz);
-                CT pHyx = SFX(cpml_term)(ps[6], ps[7], rx, ox, xb, xa, i, dEz_dx);
+                CT pHyz = SFX(cpml_term)(ps[4], ps[5], rz, oz, zb, za, k, dEx_dz, psc, pisc);
+                CT pHyx = SFX(cpml_term)(ps[6], ps[7], rx, ox, xb, xa, i, dEz_dx, psc, pisc);
                 Hy[o] = (ST)((CT)Hy[o] - dt_mu0 * ((dEx_dz * zk[k] + pHyz) - (dEz_dx * xk[i] + pHyx)));
 
-                CT pHzx = SFX(cpml_term)(ps[8], ps[9], rx, ox, xb, xa, i, dEy_dx);
-                CT pHzy = SFX(cpml_term)(ps[10], ps[11], ry, oy, yb, ya, j, dEx_dy);
+                CT pHzx = SFX(cpml_term)(ps[8], ps[9], rx, ox, xb, xa, i, dEy_dx, psc, pisc);
+                CT pHzy = SFX(cpml_term)(ps[10], ps[11], ry, oy, yb, ya, j, dEx_dy, psc, pisc);
                 Hz[o] = (ST)((CT)Hz[o] - dt_mu0 * ((dEy_dx * xk[i] + pHzx) - (dEx_dy * yk[j] + pHzy)));
             }
         }
@@ -85,7 +88,7 @@ static void SFX(orc_yee_H)(const mp_yee *g, mp_cpml *c, CT dt_mu0) {
 }
 
 /* yee_E_3d_kernel! (Kernels.jl:103-152) over 1 <= i < nx, 1 <= j < ny, 1 <= k < nz. */
-static void SFX(orc_yee_E)(const mp_yee *g, mp_cpml *c, CT dt) {
+static void SFX(orc_yee_E)(const mp_yee *g, mp_cpml *c, CT dt, float psc, float pisc) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     const int64_t sy = nx, sz = nx * ny;
     ST *Ex = (ST *)g->ex, *Ey = (ST *)g->ey, *Ez = (ST *)g->ez;
@@ -119,16 +122,16 @@ static void SFX(orc_yee_E)(const mp_yee *g, mp_cpml *c, CT dt) {
                 const int64_t oy = i + nx * (jsy + wy * k);
                 const int64_t oz = i + nx * (j + ny * jsz);
 
-                CT pDxy = SFX(cpml_term)(ps[0], ps[1], ry, oy, yb, ya, j, dHz_dy);
-                CT pDxz = SFX(cpml_term)(ps[2], ps[3], rz, oz, zb, za, k, dHy_dz);
+                CT pDxy = SFX(cpml_term)(ps[0], ps[1], ry, oy, yb, ya, j, dHz_dy, psc, pisc);
+                CT pDxz = SFX(cpml_term)(ps[2], ps[3], rz, oz, zb, za, k, dHy_dz, psc, pisc);
                 CT cx_term = (dHz_dy * yk[j] + pDxy) - (dHy_dz * zk[k] + pDxz);
 
-                CT pDyz = SFX(cpml_term)(ps[4], ps[5], rz, oz, zb, za, k, dHx_dz);
-                CT pDyx = SFX(cpml_term)(ps[6], ps[7], rx, ox, xb, xa, i, dHz_dx);
+                CT pDyz = SFX(cpml_term)(ps[4], ps[5], rz, oz, zb, za, k, dHx_dz, psc, pisc);
+                CT pDyx = SFX(cpml_term)(ps[6], ps[7], rx, ox, xb, xa, i, dHz_dx, psc, pisc);
                 CT cy_term = (dHx_dz * zk[k] + pDyz) - (dHz_dx * xk[i] + pDyx);
 
-                CT pDzx = SFX(cpml_term)(ps[8], ps[9], rx, ox, xb, xa, i, dHy_dx);
-                CT pDzy = SFX(cpml_term)(ps[10], ps[11], ry, oy, yb, ya, j, dHx_dy);
+                CT pDzx = SFX(cpml_term)(ps[8], ps[9], rx, ox, xb, xa, i, dHy_dx, psc, pisc);
+                CT pDzy = SFX(cpml_term)(ps[10], ps[11], ry, oy, yb, ya, j, dHx_dy, psc, pisc);
                 CT cz_term = (dHy_dx * xk[i] + pDzx) - (dHx_dy * yk[j] + pDzy);
 
                 CT dx_new = (CT)Dx[o] + dt * cx_term;
