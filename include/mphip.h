@@ -199,6 +199,34 @@ typedef struct mp_source {
     mp_pulse pulse;
 } mp_source;
 
+/* Learned stand-in for the sub-stepped 4TM + LLB film integrator: an MLP 11 -> 64 -> 64
+ * -> 6 with SiLU, BF16 operands on MFMA with FP32 accumulation.  The reference has no
+ * such model; it replaces the substep loop of multiphysics_kernel! (Kernels.jl:362-412).
+ *
+ * Features (raw, FP32), from the film state before the update:
+ *   Te, Tl, Ts_TM, Ts_RE, m_TM_x, m_RE_x, asinh(pabs / 1e18 W m^-3), log(dt / 1e-18 s),
+ *   gamma_e, log(Gel), T_Curie
+ * Targets (raw rates over dt, FP32):
+ *   (du_e/dt - pabs) with u_e = gamma_e Te^2 / 2, dTl/dt, dTs_TM/dt, dTs_RE/dt,
+ *   dm_TM_x/dt, dm_RE_x/dt
+ * The network sees x = (f - x_mean) * x_inv_std and predicts y = (t - y_mean) * y_inv_std.
+ *
+ * FP32 parameter layout (row-major, out x in; padding rows/columns stay zero):
+ *   W1 [64][16] @0, b1 [64] @1024, W2 [64][64] @1088, b2 [64] @5184,
+ *   W3 [16][64] @5248, b3 [16] @6272.
+ * norm (FP32): x_mean[16], x_inv_std[16], y_mean[16], y_std[16], y_inv_std[16]. */
+#define MP_MLP_IN 11
+#define MP_MLP_HIDDEN 64
+#define MP_MLP_OUT 6
+#define MP_MLP_NPARAM 6288
+#define MP_MLP_NNORM 80
+#define MP_MLP_PACKED_BYTES 22528   /* 44 BF16 MFMA fragments of 64 lanes x 4 values */
+typedef struct mp_mlp {
+    const float *params;   /* MP_MLP_NPARAM FP32 master parameters (device) */
+    void *packed;          /* MP_MLP_PACKED_BYTES, written by mp_mlp_pack (device) */
+    const float *norm;     /* MP_MLP_NNORM FP32 normalisation constants (device) */
+} mp_mlp;
+
 /* Everything one step of an FDTDState touches (Solver.jl:9-53).  Pointers to the
  * component structs above; NULL disables that part (no dispersion, no MO, no film, no
  * monitor).  t, n and last_mp_n are advanced by mp_run_steps. */
@@ -221,6 +249,12 @@ typedef struct mp_state {
     int64_t n, last_mp_n;
     int32_t multiphysics_every, subcycles, brillouin_iters, multiphysics_enabled;
     int32_t has_ade_work, record_dft;
+    /* Film update by the surrogate (mp_surrogate_step) instead of the plain integrator
+     * when non-NULL; surrogate_fallbacks (device int64, may be NULL) counts the cells
+     * that fell back to the plain integrator. */
+    const mp_mlp *surrogate;
+    double surrogate_tol;
+    int64_t *surrogate_fallbacks;
 } mp_state;
 
 /* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */
@@ -351,6 +385,48 @@ int mp_run_steps(mp_state *s, int64_t nsteps);
 
 /* Value of a pulse at time t (FP64), as the step driver evaluates it. */
 double mp_pulse_value(const mp_pulse *p, double t);
+
+/* ---- surrogate of the film integrator (no counterpart in the reference) ----------- */
+
+/* Re-pack the FP32 parameters into the BF16 MFMA fragments the kernels read
+ * (forward and transposed weights).  Call after every parameter change. */
+int mp_mlp_pack(mp_mlp *m);
+
+/* Inference on a batch: X raw features (B rows of MP_MLP_IN FP32), Y raw predicted
+ * rates (B rows of MP_MLP_OUT FP32). */
+int mp_mlp_forward(const mp_mlp *m, const float *X, float *Y, int64_t B);
+
+/* Forward + backward of the mean-squared error over the normalised targets:
+ * loss = mean_{b,o} (y_b,o - (T_b,o - y_mean_o) * y_inv_std_o)^2.  grad (device,
+ * MP_MLP_NPARAM FP32) receives dloss/dparams; *loss (device double, may be NULL) the
+ * loss.  Deterministic (fixed partition, fixed reduction order). */
+int mp_mlp_backward(const mp_mlp *m, const float *X, const float *T, int64_t B, float *grad,
+                    double *loss);
+
+/* One Adam step (decoupled weight decay wd) on the MP_MLP_NPARAM parameters; step >= 1
+ * is the bias-correction count.  Re-pack afterwards. */
+int mp_mlp_adam(float *params, const float *grad, float *m1, float *m2, int64_t step, float lr,
+                float beta1, float beta2, float eps, float wd);
+
+/* Training rows from a film state, which is not modified: for every stride-th cell
+ * (at most max_rows), the features above and the rates of the plain integrator over dt
+ * (subcycles substeps, FP64, as mp_multiphysics_step would compute them; pabs =
+ * window_sum/dt when has_pabs).  X: rows x MP_MLP_IN, T: rows x MP_MLP_OUT (device). */
+int mp_surrogate_dataset(const mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
+                         double dt, int32_t subcycles, int32_t brillouin_iters, int32_t has_pabs,
+                         int64_t stride, float *X, float *T, int64_t max_rows);
+
+/* Film update by the surrogate, in place of mp_multiphysics_step (same arguments).  Per
+ * cell: the network's rates over dt; one plain substep of dt/subcycles from the same
+ * state; if the network's state at dt/subcycles differs from that substep by more than
+ * tol (absolute in m, relative in T) the cell falls back to the plain integrator and
+ * finishes the remaining substeps, so its result is the plain one.  Accepted cells take
+ * the network's Te, Tl, Ts_TM, Ts_RE, m_TM_x, m_RE_x and extrapolate m_y, m_z from the
+ * plain substep.  tol < 0 forces the plain integrator everywhere.  *fallbacks (device
+ * int64, may be NULL) is incremented by the number of fallback cells. */
+int mp_surrogate_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
+                      const mp_mlp *m, double dt, int32_t subcycles, int32_t brillouin_iters,
+                      int32_t has_pabs, double tol, int64_t *fallbacks);
 
 /* Film reductions on device (src/sim/Monitors.jl:487-578 FilmAverage/SwitchedFraction,
  * src/physics/Magnetization.jl:280-283 norms): out[0..9] = means of m_TM_xyz, m_RE_xyz,

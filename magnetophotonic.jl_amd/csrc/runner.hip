@@ -106,8 +106,13 @@ static int one_step(mp_state *s) {
                     MP_HIP(hipStreamWaitEvent(g_ov.side, g_ov.main_done, 0));
                 }
                 hipStream_t prev = ov ? swap_stream(g_ov.side) : stream();
-                rc = mp_multiphysics_step(s->film, s->params, s->lut, s->dt * (double)window, s->subcycles,
-                                          s->brillouin_iters, 1);
+                if (s->surrogate)
+                    rc = mp_surrogate_step(s->film, s->params, s->lut, s->surrogate, s->dt * (double)window,
+                                           s->subcycles, s->brillouin_iters, 1, s->surrogate_tol,
+                                           s->surrogate_fallbacks);
+                else
+                    rc = mp_multiphysics_step(s->film, s->params, s->lut, s->dt * (double)window, s->subcycles,
+                                              s->brillouin_iters, 1);
                 if (ov) swap_stream(prev);
                 if (rc) return rc;
                 if (ov) {

@@ -73,6 +73,13 @@ class Source(C.Structure):
                 ("profile", vp), ("hcorr", f64), ("pulse", Pulse)]
 
 
+class Mlp(C.Structure):
+    _fields_ = [("params", vp), ("packed", vp), ("norm", vp)]
+
+
+MLP_NPARAM, MLP_NNORM, MLP_PACKED_BYTES = 6288, 80, 22528
+
+
 class State(C.Structure):
     _fields_ = [("yee", C.POINTER(Yee)), ("cpml", C.POINTER(Cpml)), ("ade", C.POINTER(Ade) * 3),
                 ("mo", C.POINTER(Mo)), ("mo_ea", vp), ("mo_eb", vp), ("mo_iea", vp), ("mo_ieb", vp),
@@ -81,7 +88,8 @@ class State(C.Structure):
                 ("phase", vp), ("dt", f64), ("mu0", f64), ("pabs_factor", f64), ("Q_TM", f64),
                 ("Q_RE", f64), ("t", f64), ("n", i64), ("last_mp_n", i64),
                 ("multiphysics_every", i32), ("subcycles", i32), ("brillouin_iters", i32),
-                ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32)]
+                ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32),
+                ("surrogate", C.POINTER(Mlp)), ("surrogate_tol", f64), ("surrogate_fallbacks", vp)]
 
 
 MP_PULSE_GAUSSIAN, MP_PULSE_CARRIER_GAUSSIAN, MP_PULSE_CONTINUOUS = 1, 2, 3
@@ -108,6 +116,15 @@ SIGNATURES = {
 DRIVER_SIGNATURES = {
     "run_steps": (C.c_int, [P(State), i64]),
     "pulse_value": (C.c_double, [P(Pulse), f64]),
+}
+
+SURROGATE_SIGNATURES = {
+    "mp_mlp_pack": (C.c_int, [P(Mlp)]),
+    "mp_mlp_forward": (C.c_int, [P(Mlp), vp, vp, i64]),
+    "mp_mlp_backward": (C.c_int, [P(Mlp), vp, vp, i64, vp, vp]),
+    "mp_mlp_adam": (C.c_int, [vp, vp, vp, vp, i64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "mp_surrogate_dataset": (C.c_int, [P(Film), vp, P(Lut), f64, i32, i32, i32, i64, vp, vp, i64]),
+    "mp_surrogate_step": (C.c_int, [P(Film), vp, P(Lut), P(Mlp), f64, i32, i32, i32, f64, vp]),
 }
 
 RUNTIME_SIGNATURES = {

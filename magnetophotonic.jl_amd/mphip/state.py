@@ -376,14 +376,36 @@ class FDTDState:
                 callback(self)
         return self
 
+    def attach_surrogate(self, sur, tol: float = 1e-4) -> None:
+        """Film updates by `sur` (surrogate.Surrogate) instead of the plain integrator, with
+        a plain fallback wherever the network misses one plain substep by more than tol."""
+        self._sur = sur
+        self._st.surrogate = C.pointer(sur.mlp)
+        self._st.surrogate_tol = float(tol)
+        self._st.surrogate_fallbacks = sur.fallbacks.ptr
+
+    def detach_surrogate(self) -> None:
+        self._sur = None
+        self._st.surrogate = None
+        self._st.surrogate_fallbacks = None
+
+    def _film_update(self, dt: float, subcycles: int, has_pabs: int) -> None:
+        sur = getattr(self, "_sur", None)
+        gd = self.gd_arr.ctypes.data_as(C.c_void_p)
+        if sur is None:
+            _lib.call("multiphysics_step", C.byref(self.film), gd, C.byref(self.lut), float(dt), int(subcycles),
+                      self.spec.brillouin_iters, int(has_pabs))
+        else:
+            _lib.call("surrogate_step", C.byref(self.film), gd, C.byref(self.lut), C.byref(sur.mlp), float(dt),
+                      int(subcycles), self.spec.brillouin_iters, int(has_pabs), self._st.surrogate_tol,
+                      sur.fallbacks.vp)
+
     def _consume_window(self):
         """_consume_multiphysics_window! (Solver.jl:373-383)."""
         window = self.n - self.last_mp_n
         if window <= 0:
             return
-        dt_mp = self.dt * window
-        _lib.call("multiphysics_step", C.byref(self.film), self.gd_arr.ctypes.data_as(C.c_void_p),
-                  C.byref(self.lut), dt_mp, self.spec.subcycles, self.spec.brillouin_iters, 1)
+        self._film_update(self.dt * window, self.spec.subcycles, 1)
         self.last_mp_n = self.n
 
     def flush_multiphysics(self):
@@ -405,9 +427,7 @@ class FDTDState:
         """relax_step! (Solver.jl:548-555): 4TM+LLB with no heating."""
         if self.film is None:
             return self
-        _lib.call("multiphysics_step", C.byref(self.film), self.gd_arr.ctypes.data_as(C.c_void_p),
-                  C.byref(self.lut), float(dt), int(subcycles or self.spec.subcycles),
-                  self.spec.brillouin_iters, 0)
+        self._film_update(float(dt), int(subcycles or self.spec.subcycles), 0)
         self.t += float(dt)
         return self
 
@@ -447,7 +467,8 @@ class FDTDState:
         return self
 
     def set_film(self, key: str, values: np.ndarray):
-        self.film_bufs[key].upload(np.asarray(values).astype(self.spec.compute_dtype))
+        dt = np.float64 if key in ("U_abs", "window_sum", "peak") else self.spec.compute_dtype
+        self.film_bufs[key].upload(np.asarray(values).astype(dt))
 
     def psi(self, which: str, s: int) -> np.ndarray:
         """Raw storage of CPML slab s (0..11, order xy_lo .. zy_hi) of the H ('h') or

@@ -46,6 +46,13 @@ def lib(abi):
         h.orc_brillouin_fn.argtypes = [C.c_double, C.c_double]
         h.orc_lut_lookup.restype = C.c_double
         h.orc_lut_lookup.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32]
+        vp, P = C.c_void_p, C.POINTER
+        h.orc_surrogate_dataset.restype = C.c_int
+        h.orc_surrogate_dataset.argtypes = [P(abi.Film), vp, P(abi.Lut), C.c_double, C.c_int32, C.c_int32,
+                                            C.c_int32, C.c_int64, vp, vp, C.c_int64]
+        h.orc_surrogate_apply.restype = C.c_int
+        h.orc_surrogate_apply.argtypes = [P(abi.Film), vp, P(abi.Lut), vp, C.c_double, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_double, P(C.c_int64)]
         for name in ("orc_store_reduced", "orc_load_reduced"):
             getattr(h, name).restype = None
             getattr(h, name).argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.c_float]

@@ -31,6 +31,7 @@
 
 #include "orc_physics.h"
 #include "orc_half.h"
+#include "orc_surrogate.h"
 
 #define PSI_LD(p, o) ((CT)(p)[o])
 #define PSI_ST(p, o, v) ((p)[o] = (PT)(v))
@@ -517,5 +518,43 @@ ORC_EXPORT int orc_film_reduce(const mp_film *f, double *out) {
     for (int a = 0; a < 8; ++a) out[a] = s[a] * inv;
     out[8] = neg * inv;
     out[9] = f->n > 0 ? temax : 0.0;
+    return 0;
+}
+
+/* Training rows of the surrogate (restates mp_surrogate_dataset). */
+ORC_EXPORT int orc_surrogate_dataset(const mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
+                                     double dt, int32_t subcycles, int32_t iters, int32_t has_pabs,
+                                     int64_t stride, float *X, float *T, int64_t max_rows) {
+    if (subcycles <= 0 || stride < 1) return orc_fail("orc_surrogate_dataset: bad arguments");
+    int64_t rows = (f->n + stride - 1) / stride;
+    if (rows > max_rows) rows = max_rows;
+    if (rows <= 0) return 0;
+    if (f->dtype == MP_F32)
+        orc_dataset_float(f, p, (const float *)lut->tm, (const float *)lut->re, (double)(float)lut->T_min,
+                          (double)(float)lut->inv_dT, lut->n, (double)(float)dt, subcycles, iters, has_pabs,
+                          stride, rows, X, T);
+    else
+        orc_dataset_double(f, p, (const double *)lut->tm, (const double *)lut->re, lut->T_min, lut->inv_dT,
+                           lut->n, dt, subcycles, iters, has_pabs, stride, rows, X, T);
+    return 0;
+}
+
+/* Surrogate step given the network's raw rates (f->n rows of 6, host FP32); restates
+ * the film side of mp_surrogate_step.  *fallbacks receives the fallback count. */
+ORC_EXPORT int orc_surrogate_apply(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
+                                   const float *rates, double dt, int32_t subcycles, int32_t iters,
+                                   int32_t has_pabs, double tol, int64_t *fallbacks) {
+    if (subcycles <= 0) return orc_fail("orc_surrogate_apply: subcycles must be positive");
+    int64_t fb = 0;
+    if (f->n > 0) {
+        if (f->dtype == MP_F32)
+            fb = orc_surrogate_float(f, p, (const float *)lut->tm, (const float *)lut->re,
+                                     (double)(float)lut->T_min, (double)(float)lut->inv_dT, lut->n, rates,
+                                     (double)(float)dt, subcycles, iters, has_pabs, tol);
+        else
+            fb = orc_surrogate_double(f, p, (const double *)lut->tm, (const double *)lut->re, lut->T_min,
+                                      lut->inv_dT, lut->n, rates, dt, subcycles, iters, has_pabs, tol);
+    }
+    if (fallbacks) *fallbacks = fb;
     return 0;
 }

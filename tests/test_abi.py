@@ -36,7 +36,7 @@ def test_oracle_mirrors_kernel_entry_points():
 
 STRUCTS = {"mp_yee": "Yee", "mp_cpml": "Cpml", "mp_ade": "Ade", "mp_mo": "Mo", "mp_film": "Film",
            "mp_lut": "Lut", "mp_plane": "Plane", "mp_pulse": "Pulse", "mp_source": "Source",
-           "mp_state": "State"}
+           "mp_state": "State", "mp_mlp": "Mlp"}
 
 
 def test_struct_layouts_match_header(tmp_path):
@@ -54,6 +54,10 @@ def test_struct_layouts_match_header(tmp_path):
             expect.append(getattr(cls, fname).offset)
     lines.append('printf("%zu\\n", sizeof(mp_ferri_params));')
     expect.append(8 * len(models.FIELDS))
+    for macro, value in (("MP_MLP_NPARAM", abi.MLP_NPARAM), ("MP_MLP_NNORM", abi.MLP_NNORM),
+                         ("MP_MLP_PACKED_BYTES", abi.MLP_PACKED_BYTES)):
+        lines.append(f'printf("%zu\\n", (size_t){macro});')
+        expect.append(value)
     lines.append("return 0;}")
     src = tmp_path / "layout.c"
     src.write_text("\n".join(lines))

@@ -1,0 +1,190 @@
+"""The surrogate's HIP kernels against the oracle (GPU).
+
+Network: mp_mlp_forward / mp_mlp_backward / mp_mlp_adam against oracle/mlp.py in the
+kernels' arithmetic (BF16 operands, FP32 accumulation).  The two differ only in FP32
+accumulation order and last-ULP exp, so outputs agree to ~1e-5 and the gradient to
+~1e-4 relative L2.
+
+Film side: mp_surrogate_dataset and mp_surrogate_step against orc_surrogate_dataset /
+orc_surrogate_apply on identical film states; with a negative tolerance the surrogate
+step is the plain integrator, bit for bit.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(B, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(B, 11)) * np.array([400, 300, 300, 300, 0.5, 0.5, 2, 1, 0, 0, 0]) + \
+        np.array([1000, 400, 500, 500, 0.2, -0.2, 3, 2, 300, 40, 540])
+    T = np.tanh(X[:, :6] / np.array([1500, 800, 800, 800, 1, 1])) * np.array([1e20, 1e12, 1e13, 1e13, 1e11, 1e11])
+    T += rng.normal(size=T.shape) * np.array([1e18, 1e10, 1e11, 1e11, 1e9, 1e9])
+    return X.astype(np.float32), T.astype(np.float32)
+
+
+def _net(seed=0):
+    import mlp
+    from mphip.surrogate import Surrogate
+    X, T = _rows(4096, seed)
+    norm = mlp.make_norm(X, T)
+    p = mlp.init_params(seed)
+    rng = np.random.default_rng(seed + 1)
+    p[mlp.OB1:mlp.OW2] = rng.normal(size=64) * 0.2
+    p[mlp.OB2:mlp.OW3] = rng.normal(size=64) * 0.2
+    p[mlp.OB3:mlp.OB3 + 6] = rng.normal(size=6) * 0.2
+    return Surrogate(params=p, norm=norm), p, norm
+
+
+@pytest.mark.parametrize("B", [1, 77, 1000, 65536 + 5])
+def test_mlp_forward_matches_oracle(B):
+    import mlp
+    from mphip.state import DeviceBuffer
+    print(f"case mlp forward B={B}", flush=True)
+    sur, p, norm = _net()
+    X, _ = _rows(B, seed=B)
+    Y = sur.forward(DeviceBuffer.from_array(X), B).download(np.float32).reshape(-1, 6)[:B]
+    want = mlp.forward(p, norm, X)
+    ys = norm[mlp.N_YS:mlp.N_YS + 6]
+    err = np.abs(Y - want) / ys          # in units of the target scale
+    assert np.median(err) < 1e-5
+    assert err.max() < 2e-2
+    sur.free()
+
+
+@pytest.mark.parametrize("B", [16, 333, 4099, 200000])
+def test_mlp_gradient_matches_oracle(B):
+    import mlp
+    from mphip.state import DeviceBuffer
+    print(f"case mlp gradient B={B}", flush=True)
+    sur, p, norm = _net(1)
+    X, T = _rows(B, seed=B + 7)
+    Xd, Td = DeviceBuffer.from_array(X), DeviceBuffer.from_array(T)
+    sur.backward(Xd.ptr, Td.ptr, B)
+    g = sur.grad.download(np.float32)
+    loss, want = mlp.loss_and_grad(p, norm, X, T)
+    assert abs(sur.loss() - loss) < 1e-5 * loss
+    assert np.linalg.norm(g - want) < 1e-3 * np.linalg.norm(want)
+    W1, _, W2, _, W3, b3 = mlp.unpack(g)
+    assert not np.any(W1[:, 11:]) and not np.any(W3[6:]) and not np.any(b3[6:])
+    # deterministic: the same call twice gives the same bits
+    sur.backward(Xd.ptr, Td.ptr, B)
+    assert np.array_equal(sur.grad.download(np.float32), g)
+    sur.free()
+
+
+def test_adam_matches_oracle():
+    import mlp
+    from mphip.state import DeviceBuffer
+    sur, p, norm = _net(2)
+    rng = np.random.default_rng(9)
+    m1 = np.zeros(mlp.NPARAM, np.float32)
+    m2 = np.zeros(mlp.NPARAM, np.float32)
+    q = p.copy()
+    for step in range(1, 4):
+        g = rng.normal(size=mlp.NPARAM).astype(np.float32) * np.float32(1e-2)
+        sur.grad.upload(g)
+        sur.adam(1e-3, wd=1e-4)
+        q, m1, m2 = mlp.adam(q, g, m1, m2, step, 1e-3, wd=1e-4)
+    np.testing.assert_allclose(sur.get_params(), q, rtol=2e-6, atol=1e-9)
+    sur.free()
+
+
+def test_training_lowers_loss():
+    from mphip.surrogate import Dataset, Surrogate, make_norm
+    X, T = _rows(262144, seed=11)
+    sur = Surrogate(norm=make_norm(X, T), seed=3)
+    data = Dataset(X, T)
+    l0 = sur.evaluate(data)
+    sur.train(data, steps=400, batch=16384, lr=3e-3, lr_end=3e-4)
+    l1 = sur.evaluate(data)
+    print(f"loss {l0:.4e} -> {l1:.4e}", flush=True)
+    assert l1 < 0.05 * l0
+    sur.free()
+    data.free()
+
+
+def _film_pair(seed, dtype="f32"):
+    import driver
+    from mphip import configs
+    from mphip.state import FDTDState
+    spec, _ = configs.coupled_pump(nx=24, ny=20, nz=32, steps=10, fluence_mj_cm2=30.0, npml=4, src_z=20e-9,
+                                   n_planes=0, enable_mo=False, dtype=dtype)
+    gpu, ref = FDTDState(spec), driver.OracleState(spec)
+    rng = np.random.default_rng(seed)
+    N = spec.n_film
+    vals = dict(Te=rng.uniform(300, 2500, N), Tl=rng.uniform(300, 700, N), Ts_TM=rng.uniform(300, 900, N),
+                Ts_RE=rng.uniform(300, 900, N), m_TM_x=rng.uniform(-0.9, 0.9, N), m_RE_x=rng.uniform(-0.9, 0.9, N),
+                window_sum=rng.uniform(0, 1e22, N) * spec.dt)
+    for k, v in vals.items():
+        gpu.set_film(k, v)
+        ref.set_film(k, v) if k != "window_sum" else ref.fa[k].__setitem__(slice(None), v)
+    return gpu, ref
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_surrogate_negative_tol_is_plain_bitwise(dtype):
+    from mphip import _lib
+    print(f"case surrogate tol<0 == plain {dtype}", flush=True)
+    sur, _, _ = _net(4)
+    a, _ = _film_pair(5, dtype)
+    b, _ = _film_pair(5, dtype)
+    gd = a.gd_arr.ctypes.data_as(C.c_void_p)
+    sur.reset_fallbacks()
+    _lib.call("surrogate_step", C.byref(a.film), gd, C.byref(a.lut), C.byref(sur.mlp), a.dt, 4,
+              a.spec.brillouin_iters, 1, -1.0, sur.fallbacks.vp)
+    _lib.call("multiphysics_step", C.byref(b.film), b.gd_arr.ctypes.data_as(C.c_void_p), C.byref(b.lut), b.dt, 4,
+              b.spec.brillouin_iters, 1)
+    fa, fb = a.film_state(), b.film_state()
+    for k in ("Te", "Tl", "Ts_TM", "Ts_RE", "m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z", "window_sum"):
+        assert np.array_equal(fa[k], fb[k]), k
+    assert sur.fallback_count() == a.spec.n_film
+    sur.free()
+
+
+def test_dataset_matches_oracle():
+    from mphip.surrogate import film_rows
+    print("case surrogate dataset vs oracle", flush=True)
+    gpu, ref = _film_pair(6)
+    for dt, sub, hp in ((gpu.dt, 4, True), (5e-15, 10, False)):
+        X, T = film_rows(gpu, dt, sub, hp, stride=3)
+        rows = X.shape[0]
+        Xr = np.zeros((rows, 11), np.float32)
+        Tr = np.zeros((rows, 6), np.float32)
+        assert ref.L.orc_surrogate_dataset(C.byref(ref.film), ref.gd_arr.ctypes.data, C.byref(ref.lut), dt, sub,
+                                           ref.spec.brillouin_iters, 1 if hp else 0, 3, Xr.ctypes.data,
+                                           Tr.ctypes.data, rows) == 0
+        np.testing.assert_allclose(X, Xr, rtol=1e-6, atol=0)
+        scale = np.abs(Tr).max(0)
+        assert np.all(np.abs(T - Tr) <= 1e-6 * scale + 1e-6 * np.abs(Tr)), (dt, np.abs(T - Tr).max(0) / scale)
+
+
+def test_surrogate_step_matches_oracle():
+    from mphip import _lib
+    from mphip.state import DeviceBuffer
+    from mphip.surrogate import film_rows
+    print("case surrogate step vs oracle", flush=True)
+    sur, _, _ = _net(8)
+    gpu, ref = _film_pair(9)
+    N = gpu.spec.n_film
+    X, _ = film_rows(gpu, gpu.dt, 4, True)
+    rates = sur.forward(DeviceBuffer.from_array(X), N).download(np.float32).reshape(-1, 6)[:N].copy()
+    for tol in (1e9, 1e-4):
+        a, r = _film_pair(9)
+        sur.reset_fallbacks()
+        _lib.call("surrogate_step", C.byref(a.film), a.gd_arr.ctypes.data_as(C.c_void_p), C.byref(a.lut),
+                  C.byref(sur.mlp), a.dt, 4, a.spec.brillouin_iters, 1, tol, sur.fallbacks.vp)
+        fb = C.c_int64(0)
+        assert r.L.orc_surrogate_apply(C.byref(r.film), r.gd_arr.ctypes.data, C.byref(r.lut), rates.ctypes.data,
+                                       r.dt, 4, r.spec.brillouin_iters, 1, tol, C.byref(fb)) == 0
+        fa, fr = a.film_state(), r.film_state()
+        print(f"tol {tol}: fallbacks gpu {sur.fallback_count()} oracle {fb.value} of {N}", flush=True)
+        assert abs(sur.fallback_count() - fb.value) <= max(2, N // 1000)
+        for k in ("Te", "Tl", "Ts_TM", "Ts_RE"):
+            assert np.median(np.abs(fa[k] / fr[k] - 1)) < 1e-6, k
+        for k in ("m_TM_x", "m_RE_x", "m_TM_y", "m_RE_z"):
+            assert np.median(np.abs(fa[k].astype(np.float64) - fr[k])) < 1e-6, k
+    sur.free()
