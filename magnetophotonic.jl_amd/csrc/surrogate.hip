@@ -478,6 +478,40 @@ __global__ __launch_bounds__(128) void dataset_kernel(
     t[5] = (float)((m[3] - mR0) / dt);
 }
 
+// A cell that failed the check: its state after the first plain substep (FP64, so the
+// remaining substeps continue exactly as the plain loop would), its absorbed power and
+// its index.  Fallback cells are compacted into a list and finished by a second kernel,
+// so the substep loop runs on full waves instead of diverging inside the first.
+struct FallbackRec {
+    double s[11];  // te, tl, tsT, tsR, m[6], pabs
+    int64_t cell;
+};
+
+__global__ void reset_count_kernel(unsigned long long *c) { *c = 0ull; }
+
+template <typename L>
+__global__ __launch_bounds__(128) void surrogate_fallback_kernel(
+    L *__restrict__ Te, L *__restrict__ Tl, L *__restrict__ TsT, L *__restrict__ TsR, L *__restrict__ mTx,
+    L *__restrict__ mTy, L *__restrict__ mTz, L *__restrict__ mRx, L *__restrict__ mRy, L *__restrict__ mRz,
+    const FallbackRec *__restrict__ recs, const unsigned long long *__restrict__ nrec, const mp_ferri_params gd,
+    const L *__restrict__ tm_lut, const L *__restrict__ re_lut, double T_min, double inv_dT, int32_t lut_N,
+    double sub_dt, int subcycles, int iters) {
+    const unsigned long long cnt = *nrec;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
+         i += (unsigned long long)gridDim.x * blockDim.x) {
+        const FallbackRec &r = recs[i];
+        double te = r.s[0], tl = r.s[1], tsT = r.s[2], tsR = r.s[3];
+        double m[6] = {r.s[4], r.s[5], r.s[6], r.s[7], r.s[8], r.s[9]};
+        const double pabs = r.s[10];
+        const int64_t n = r.cell;
+        for (int k = 1; k < subcycles; ++k)
+            plain_substep<L>(te, tl, tsT, tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
+        Te[n] = (L)te; Tl[n] = (L)tl; TsT[n] = (L)tsT; TsR[n] = (L)tsR;
+        mTx[n] = (L)m[0]; mTy[n] = (L)m[1]; mTz[n] = (L)m[2];
+        mRx[n] = (L)m[3]; mRy[n] = (L)m[4]; mRz[n] = (L)m[5];
+    }
+}
+
 template <typename L, bool HAS_PABS>
 __global__ __launch_bounds__(256) void surrogate_kernel(
     L *__restrict__ Te, L *__restrict__ Tl, L *__restrict__ TsT, L *__restrict__ TsR, L *__restrict__ mTx,
@@ -485,7 +519,8 @@ __global__ __launch_bounds__(256) void surrogate_kernel(
     double *__restrict__ window, const mp_ferri_params gd, const L *__restrict__ tm_lut,
     const L *__restrict__ re_lut, double T_min, double inv_dT, int32_t lut_N, double dt, int subcycles,
     int iters, int64_t N, const uint2 *__restrict__ packed, const float *__restrict__ P,
-    const float *__restrict__ norm, double tol, unsigned long long *__restrict__ fallbacks) {
+    const float *__restrict__ norm, double tol, unsigned long long *__restrict__ fallbacks,
+    FallbackRec *__restrict__ recs, unsigned long long *__restrict__ nrec) {
     __shared__ InferLds s;
     __shared__ __align__(16) uint16_t xs[4][64][16];
     __shared__ float ys[4][64][8];
@@ -520,56 +555,71 @@ __global__ __launch_bounds__(256) void surrogate_kernel(
     const double sub_dt = dt / (double)subcycles;
     plain_substep<L>(te, tl, tsT, tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
 
+    // Trapezoid check of the network's whole step: with r0 the plain one-substep rate at
+    // the start and r1 the plain one-substep rate at the network's end state, the step
+    // must match dt/2 (r0 + r1) to tol (absolute in m, relative in T).
     bool accept = tol >= 0.0;
     const double hg = 0.5 * gd.gamma_e;
     const double ue0 = hg * (te0 * te0);
-    double nte = 0, ntl = 0, ntsT = 0, ntsR = 0, nmT = 0, nmR = 0;
+    const double S = (double)subcycles;
+    double e_te = 0, e_tl = 0, e_tsT = 0, e_tsR = 0, e_m[6] = {0, 0, 0, 0, 0, 0};
     if (accept) {
-        // the network's state after one substep, against the plain substep
-        const double ueh = jl_max(hg, ue0 + sub_dt * (pabs + (double)r[0]));
-        const double teh = sqrt(ueh / hg);
-        double res = __builtin_fabs(teh - te) / te;
-        res = jl_max(res, __builtin_fabs(tl0 + sub_dt * (double)r[1] - tl) / tl);
-        res = jl_max(res, __builtin_fabs(tsT0 + sub_dt * (double)r[2] - tsT) / tsT);
-        res = jl_max(res, __builtin_fabs(tsR0 + sub_dt * (double)r[3] - tsR) / tsR);
-        res = jl_max(res, __builtin_fabs(m0[0] + sub_dt * (double)r[4] - m[0]));
-        res = jl_max(res, __builtin_fabs(m0[3] + sub_dt * (double)r[5] - m[3]));
+        e_te = sqrt(jl_max(hg, ue0 + dt * (pabs + (double)r[0])) / hg);
+        e_tl = jl_max(1.0, tl0 + dt * (double)r[1]);
+        e_tsT = jl_max(1.0, tsT0 + dt * (double)r[2]);
+        e_tsR = jl_max(1.0, tsR0 + dt * (double)r[3]);
+        e_m[0] = m0[0] + dt * (double)r[4];
+        e_m[3] = m0[3] + dt * (double)r[5];
+        e_m[1] = m0[1] + S * (m[1] - m0[1]);
+        e_m[2] = m0[2] + S * (m[2] - m0[2]);
+        e_m[4] = m0[4] + S * (m[4] - m0[4]);
+        e_m[5] = m0[5] + S * (m[5] - m0[5]);
+        accept = __builtin_isfinite(e_te) && __builtin_isfinite(e_tl) && __builtin_isfinite(e_tsT) &&
+                 __builtin_isfinite(e_tsR) && __builtin_fabs(e_m[0]) <= 1.0 && __builtin_fabs(e_m[3]) <= 1.0;
+    }
+    if (accept) {
+        double b_te = e_te, b_tl = e_tl, b_tsT = e_tsT, b_tsR = e_tsR;
+        double b_m[6] = {e_m[0], e_m[1], e_m[2], e_m[3], e_m[4], e_m[5]};
+        plain_substep<L>(b_te, b_tl, b_tsT, b_tsR, b_m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
+        const double hS = 0.5 * S;
+        double res = __builtin_fabs(e_te - te0 - hS * ((te - te0) + (b_te - e_te))) / e_te;
+        res = jl_max(res, __builtin_fabs(e_tl - tl0 - hS * ((tl - tl0) + (b_tl - e_tl))) / e_tl);
+        res = jl_max(res, __builtin_fabs(e_tsT - tsT0 - hS * ((tsT - tsT0) + (b_tsT - e_tsT))) / e_tsT);
+        res = jl_max(res, __builtin_fabs(e_tsR - tsR0 - hS * ((tsR - tsR0) + (b_tsR - e_tsR))) / e_tsR);
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            res = jl_max(res, __builtin_fabs(e_m[c] - m0[c] - hS * ((m[c] - m0[c]) + (b_m[c] - e_m[c]))));
         accept = res <= tol;  // false for NaN
-        if (accept) {
-            const double ue = jl_max(hg, ue0 + dt * (pabs + (double)r[0]));
-            nte = sqrt(ue / hg);
-            ntl = jl_max(1.0, tl0 + dt * (double)r[1]);
-            ntsT = jl_max(1.0, tsT0 + dt * (double)r[2]);
-            ntsR = jl_max(1.0, tsR0 + dt * (double)r[3]);
-            nmT = m0[0] + dt * (double)r[4];
-            nmR = m0[3] + dt * (double)r[5];
-            accept = __builtin_isfinite(nte) && __builtin_isfinite(ntl) && __builtin_isfinite(ntsT) &&
-                     __builtin_isfinite(ntsR) && __builtin_fabs(nmT) <= 1.0 && __builtin_fabs(nmR) <= 1.0;
+    }
+    const unsigned long long mask = __ballot(!accept);
+    if (mask) {
+        // one list slot range per wave, lanes in order inside it
+        const int leader = __ffsll((long long)__ballot(1)) - 1;
+        uint32_t lo = 0, hi = 0;
+        if (l == leader) {
+            const unsigned long long base = atomicAdd(nrec, (unsigned long long)__popcll(mask));
+            lo = (uint32_t)base;
+            hi = (uint32_t)(base >> 32);
+            if (fallbacks) atomicAdd(fallbacks, (unsigned long long)__popcll(mask));
+        }
+        lo = __shfl(lo, leader);
+        hi = __shfl(hi, leader);
+        if (!accept) {
+            const unsigned long long slot = (((unsigned long long)hi << 32) | lo) +
+                                            (unsigned long long)__popcll(mask & ((1ull << l) - 1ull));
+            FallbackRec &rec = recs[slot];
+            rec.s[0] = te; rec.s[1] = tl; rec.s[2] = tsT; rec.s[3] = tsR;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) rec.s[4 + c] = m[c];
+            rec.s[10] = pabs;
+            rec.cell = n;
         }
     }
     if (accept) {
-        const double S = (double)subcycles;
-        te = nte;
-        tl = ntl;
-        tsT = ntsT;
-        tsR = ntsR;
-        m[1] = m0[1] + S * (m[1] - m0[1]);
-        m[2] = m0[2] + S * (m[2] - m0[2]);
-        m[4] = m0[4] + S * (m[4] - m0[4]);
-        m[5] = m0[5] + S * (m[5] - m0[5]);
-        m[0] = nmT;
-        m[3] = nmR;
-    } else {
-        for (int k = 1; k < subcycles; ++k)
-            plain_substep<L>(te, tl, tsT, tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
+        Te[n] = (L)e_te; Tl[n] = (L)e_tl; TsT[n] = (L)e_tsT; TsR[n] = (L)e_tsR;
+        mTx[n] = (L)e_m[0]; mTy[n] = (L)e_m[1]; mTz[n] = (L)e_m[2];
+        mRx[n] = (L)e_m[3]; mRy[n] = (L)e_m[4]; mRz[n] = (L)e_m[5];
     }
-    if (fallbacks) {
-        const unsigned long long mask = __ballot(!accept);
-        if (l == __ffsll((long long)__ballot(1)) - 1 && mask) atomicAdd(fallbacks, (unsigned long long)__popcll(mask));
-    }
-    Te[n] = (L)te; Tl[n] = (L)tl; TsT[n] = (L)tsT; TsR[n] = (L)tsR;
-    mTx[n] = (L)m[0]; mTy[n] = (L)m[1]; mTz[n] = (L)m[2];
-    mRx[n] = (L)m[3]; mRy[n] = (L)m[4]; mRz[n] = (L)m[5];
 }
 
 // ---- launch helpers --------------------------------------------------------------------
@@ -598,20 +648,50 @@ static int launch_dataset(const mp_film *f, const mp_ferri_params *p, const mp_l
     return MP_OK;
 }
 
+static FallbackRec *g_recs = nullptr;
+static int64_t g_recs_cap = 0;
+static unsigned long long *g_nrec = nullptr;
+
+static int ensure_fallback_list(int64_t n) {
+    if (!g_nrec) MP_HIP(hipMalloc((void **)&g_nrec, sizeof(unsigned long long)));
+    if (g_recs_cap < n) {
+        if (g_recs) MP_HIP(hipFree(g_recs));
+        g_recs = nullptr;
+        g_recs_cap = 0;
+        MP_HIP(hipMalloc((void **)&g_recs, sizeof(FallbackRec) * (size_t)n));
+        g_recs_cap = n;
+    }
+    return MP_OK;
+}
+
 template <typename L>
 static int launch_surrogate(mp_film *f, const mp_ferri_params *p, const mp_lut *lut, const mp_mlp *m, double dt,
                             int subcycles, int iters, int has_pabs, double tol, int64_t *fallbacks) {
     const int64_t N = f->n;
     const double dtc = (double)(L)dt, tmin = (double)(L)lut->T_min, invdt = (double)(L)lut->inv_dT;
+    if (int e = ensure_fallback_list(N)) return e;
+    reset_count_kernel<<<1, 1, 0, stream()>>>(g_nrec);
+    MP_LAUNCH_CHECK("reset_count_kernel");
     KernelTimer t("surrogate");
 #define SG_ARGS (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, \
                 (L *)f->m_TM_z, (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, f->window_sum, *p,    \
                 (const L *)lut->tm, (const L *)lut->re, tmin, invdt, lut->n, dtc, subcycles, iters, N, \
-                (const uint2 *)m->packed, m->params, m->norm, tol, (unsigned long long *)fallbacks
+                (const uint2 *)m->packed, m->params, m->norm, tol, (unsigned long long *)fallbacks, g_recs, g_nrec
     if (has_pabs) surrogate_kernel<L, true><<<cdiv(N, 256), 256, 0, stream()>>>(SG_ARGS);
     else surrogate_kernel<L, false><<<cdiv(N, 256), 256, 0, stream()>>>(SG_ARGS);
 #undef SG_ARGS
     MP_LAUNCH_CHECK("surrogate_kernel");
+    {
+        // the cells that failed the check finish their remaining substeps (none when
+        // subcycles == 1: the first plain substep is then the whole plain step)
+        KernelTimer t2("surrogate_fallback");
+        const int64_t blocks = std::min<int64_t>(cdiv(N, 128), 4096);
+        surrogate_fallback_kernel<L><<<blocks, 128, 0, stream()>>>(
+            (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, (L *)f->m_TM_z,
+            (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, g_recs, g_nrec, *p, (const L *)lut->tm,
+            (const L *)lut->re, tmin, invdt, lut->n, dtc / (double)subcycles, subcycles, iters);
+        MP_LAUNCH_CHECK("surrogate_fallback_kernel");
+    }
     return MP_OK;
 }
 

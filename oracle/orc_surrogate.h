@@ -91,44 +91,47 @@ static int64_t orc_surrogate_##LT(mp_film *f, const mp_ferri_params *gd, const L
         const double sub_dt = dt / (double)subcycles;                                         \
         orc_plain_substep_##LT(&te, &tl, &tsT, &tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, \
                                lut_N, sub_dt, iters);                                         \
+        /* trapezoid check of the network's step against one plain substep at each end */     \
         int accept = tol >= 0.0;                                                              \
         const double hg = 0.5 * gd->gamma_e;                                                  \
         const double ue0 = hg * (te0 * te0);                                                  \
-        double nte = 0, ntl = 0, ntsT = 0, ntsR = 0, nmT = 0, nmR = 0;                        \
+        const double S = (double)subcycles;                                                   \
+        double e_te = 0, e_tl = 0, e_tsT = 0, e_tsR = 0, e_m[6] = {0, 0, 0, 0, 0, 0};         \
         if (accept) {                                                                         \
-            const double ueh = jl_max(hg, ue0 + sub_dt * (pabs + (double)r[0]));              \
-            const double teh = sqrt(ueh / hg);                                                \
-            double res = fabs(teh - te) / te;                                                 \
-            res = jl_max(res, fabs(tl0 + sub_dt * (double)r[1] - tl) / tl);                   \
-            res = jl_max(res, fabs(tsT0 + sub_dt * (double)r[2] - tsT) / tsT);                \
-            res = jl_max(res, fabs(tsR0 + sub_dt * (double)r[3] - tsR) / tsR);                \
-            res = jl_max(res, fabs(m0[0] + sub_dt * (double)r[4] - m[0]));                    \
-            res = jl_max(res, fabs(m0[3] + sub_dt * (double)r[5] - m[3]));                    \
-            accept = res <= tol;                                                              \
-            if (accept) {                                                                     \
-                const double ue = jl_max(hg, ue0 + dt * (pabs + (double)r[0]));               \
-                nte = sqrt(ue / hg);                                                          \
-                ntl = jl_max(1.0, tl0 + dt * (double)r[1]);                                   \
-                ntsT = jl_max(1.0, tsT0 + dt * (double)r[2]);                                 \
-                ntsR = jl_max(1.0, tsR0 + dt * (double)r[3]);                                 \
-                nmT = m0[0] + dt * (double)r[4];                                              \
-                nmR = m0[3] + dt * (double)r[5];                                              \
-                accept = isfinite(nte) && isfinite(ntl) && isfinite(ntsT) && isfinite(ntsR) && \
-                         fabs(nmT) <= 1.0 && fabs(nmR) <= 1.0;                                \
-            }                                                                                 \
+            e_te = sqrt(jl_max(hg, ue0 + dt * (pabs + (double)r[0])) / hg);                   \
+            e_tl = jl_max(1.0, tl0 + dt * (double)r[1]);                                      \
+            e_tsT = jl_max(1.0, tsT0 + dt * (double)r[2]);                                    \
+            e_tsR = jl_max(1.0, tsR0 + dt * (double)r[3]);                                    \
+            e_m[0] = m0[0] + dt * (double)r[4];                                               \
+            e_m[3] = m0[3] + dt * (double)r[5];                                               \
+            e_m[1] = m0[1] + S * (m[1] - m0[1]);                                              \
+            e_m[2] = m0[2] + S * (m[2] - m0[2]);                                              \
+            e_m[4] = m0[4] + S * (m[4] - m0[4]);                                              \
+            e_m[5] = m0[5] + S * (m[5] - m0[5]);                                              \
+            accept = isfinite(e_te) && isfinite(e_tl) && isfinite(e_tsT) && isfinite(e_tsR) &&\
+                     fabs(e_m[0]) <= 1.0 && fabs(e_m[3]) <= 1.0;                              \
         }                                                                                     \
         if (accept) {                                                                         \
-            const double S = (double)subcycles;                                               \
-            te = nte; tl = ntl; tsT = ntsT; tsR = ntsR;                                       \
-            m[1] = m0[1] + S * (m[1] - m0[1]);                                                \
-            m[2] = m0[2] + S * (m[2] - m0[2]);                                                \
-            m[4] = m0[4] + S * (m[4] - m0[4]);                                                \
-            m[5] = m0[5] + S * (m[5] - m0[5]);                                                \
-            m[0] = nmT;                                                                       \
-            m[3] = nmR;                                                                       \
+            double b_te = e_te, b_tl = e_tl, b_tsT = e_tsT, b_tsR = e_tsR;                    \
+            double b_m[6];                                                                    \
+            for (int c = 0; c < 6; ++c) b_m[c] = e_m[c];                                      \
+            orc_plain_substep_##LT(&b_te, &b_tl, &b_tsT, &b_tsR, b_m, pabs, gd, tm_lut, re_lut, T_min,\
+                                   inv_dT, lut_N, sub_dt, iters);                             \
+            const double hS = 0.5 * S;                                                        \
+            double res = fabs(e_te - te0 - hS * ((te - te0) + (b_te - e_te))) / e_te;         \
+            res = jl_max(res, fabs(e_tl - tl0 - hS * ((tl - tl0) + (b_tl - e_tl))) / e_tl);   \
+            res = jl_max(res, fabs(e_tsT - tsT0 - hS * ((tsT - tsT0) + (b_tsT - e_tsT))) / e_tsT);\
+            res = jl_max(res, fabs(e_tsR - tsR0 - hS * ((tsR - tsR0) + (b_tsR - e_tsR))) / e_tsR);\
+            for (int c = 0; c < 6; ++c)                                                       \
+                res = jl_max(res, fabs(e_m[c] - m0[c] - hS * ((m[c] - m0[c]) + (b_m[c] - e_m[c]))));\
+            accept = res <= tol;                                                              \
+        }                                                                                     \
+        if (accept) {                                                                         \
+            te = e_te; tl = e_tl; tsT = e_tsT; tsR = e_tsR;                                   \
+            for (int c = 0; c < 6; ++c) m[c] = e_m[c];                                        \
         } else {                                                                              \
             for (int k = 1; k < subcycles; ++k)                                               \
-                orc_plain_substep_##LT(&te, &tl, &tsT, &tsR, m, pabs, gd, tm_lut, re_lut, T_min, \
+                orc_plain_substep_##LT(&te, &tl, &tsT, &tsR, m, pabs, gd, tm_lut, re_lut, T_min,\
                                        inv_dT, lut_N, sub_dt, iters);                         \
             fallbacks += 1;                                                                   \
         }                                                                                     \

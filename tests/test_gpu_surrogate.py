@@ -188,3 +188,89 @@ def test_surrogate_step_matches_oracle():
         for k in ("m_TM_x", "m_RE_x", "m_TM_y", "m_RE_z"):
             assert np.median(np.abs(fa[k].astype(np.float64) - fr[k])) < 1e-6, k
     sur.free()
+
+
+def test_trained_surrogate_gain_on_relax():
+    """What the trained network gains over the method without it.
+
+    Rows: the relax phase (5 fs film steps, 10 plain substeps, no heating) after the
+    96 x 96 x 64 coupled pump at 35, 50, 70 and 85 mJ/cm^2, every 5th step, every 2nd film
+    cell.  The network is trained on the GPU (30 000 Adam steps) and evaluated on the
+    held-out 60 mJ/cm^2 run against the FP64 oracle's film-mean magnetisation.  The
+    surrogate step costs the network plus 2 plain substeps per accepted cell; without
+    the network, 1 or 2 plain substeps per 5 fs step switch the wrong cells.  Plain with
+    3 substeps is printed beside it: at this step size it is both cheaper and more
+    accurate than the surrogate (measured, not asserted).
+    """
+    import time
+
+    import driver
+    from mphip import _lib, configs
+    from mphip import surrogate as S
+    from mphip.state import FDTDState
+    print("case trained surrogate vs plain on the relax phase 96x96x64", flush=True)
+    pump = dict(nx=96, ny=96, nz=64, npml=8, src_z=60e-9, n_planes=0)
+    psteps, relax_n, rdt, rsub = 1500, 600, 5e-15, 10
+    Xs, Ts = [], []
+    for fl in (35.0, 50.0, 70.0, 85.0):
+        spec, _ = configs.coupled_pump(dtype="f32", steps=psteps, fluence_mj_cm2=fl, **pump)
+        st = FDTDState(spec)
+        st.run(psteps)
+        st.flush_multiphysics()
+        for r in range(relax_n):
+            if r % 5 == 0:
+                X, T = S.film_rows(st, rdt, rsub, False, 2)
+                Xs.append(X)
+                Ts.append(T)
+            st.relax_step(rdt, subcycles=rsub)
+        st.free()
+    X, T = S.shuffled(np.concatenate(Xs), np.concatenate(Ts), 0)
+    sur = S.Surrogate(norm=S.make_norm(X[:200000], T[:200000]), seed=1)
+    data = S.Dataset(X, T)
+    sur.train(data, 30000, batch=65536, lr=3e-3, lr_end=1e-5)
+    data.free()
+
+    spec, _ = configs.coupled_pump(dtype="f64", steps=psteps, fluence_mj_cm2=60.0, **pump)
+    ref = driver.OracleState(spec)
+    ref.run(psteps)
+    ref.flush_multiphysics()
+    for _ in range(relax_n):
+        ref.relax_step(rdt, subcycles=rsub)
+    rf = ref.film_state()
+    spec, _ = configs.coupled_pump(dtype="f32", steps=psteps, fluence_mj_cm2=60.0, **pump)
+    st = FDTDState(spec)
+    st.run(psteps)
+    st.flush_multiphysics()
+    start = st.film_state()
+    names = ("Te", "Tl", "Ts_TM", "Ts_RE", "m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z")
+
+    def relax(sub=rsub, tol=None):
+        for k in names:
+            st.set_film(k, start[k])
+        if tol is not None:
+            st.attach_surrogate(sur, tol)
+            sur.reset_fallbacks()
+        _lib.call("synchronize")
+        t0 = time.time()
+        for _ in range(relax_n):
+            st.relax_step(rdt, subcycles=sub)
+        _lib.call("synchronize")
+        ms = 1e3 * (time.time() - t0)
+        f = st.film_state()
+        fb = sur.fallback_count() / (spec.n_film * relax_n) if tol is not None else 0.0
+        st.detach_surrogate()
+        err = max(abs(float(f[k].astype(np.float64).mean() - rf[k].mean())) for k in ("m_TM_x", "m_RE_x"))
+        return err, ms, fb
+
+    res = {"plain10": relax(), "plain1": relax(sub=1), "plain2": relax(sub=2), "plain3": relax(sub=3),
+           "surrogate_tol1e-3": relax(tol=1e-3), "surrogate_tol1e-4": relax(tol=1e-4)}
+    for k, (e, ms, fb) in res.items():
+        print(f"{k}: film-mean |dm| {e:.2e}  relax {ms:.1f} ms  fallback {fb:.3f}", flush=True)
+    assert res["plain10"][0] < 1e-5
+    e_sur = res["surrogate_tol1e-3"][0]
+    assert e_sur < res["plain1"][0] / 100
+    assert e_sur < res["plain2"][0] / 5
+    assert res["surrogate_tol1e-4"][0] < 5e-3
+    assert res["surrogate_tol1e-3"][2] < 0.5
+    st.free()
+    sur.free()
