@@ -49,7 +49,12 @@ __device__ __forceinline__ v4s as_v4s(uint2 u) {
     r[3] = (short)(u.y >> 16);
     return r;
 }
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+// Logistic function for the SiLU activations, on the native exp2 and reciprocal
+// instructions (about 1 ulp each): the result is rounded to BF16 before it is used as
+// an operand, so a correctly rounded exp and division would buy nothing.
+__device__ __forceinline__ float sigm(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504088896341f * z));
+}
 
 // Raw features of one cell (the order of mp_mlp in mphip.h).
 __device__ __forceinline__ void cell_features(float f[MP_MLP_IN], double te, double tl, double tsT,

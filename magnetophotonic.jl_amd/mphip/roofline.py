@@ -74,3 +74,16 @@ def step_bytes(spec, planes=(), nb: int = 0) -> dict:
         kb["dft_accumulate"] = plane_bytes(spec, planes, nb)
     kb["total"] = int(sum(v for v in kb.values()))
     return kb
+
+
+# HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE x 2.0 + WRITE_SIZE,
+# KiB per dispatch; the factors are what the counters read for a plain copy of known size
+# on gfx950), in runs separate from the timed one: profiles/yee_E_coupled_pump_512x512x256_pmc.md.
+MEASURED_TRAFFIC = {
+    ("coupled_pump_512x512x256", "plain"): {"yee_E": 4349117132, "yee_H": 2762425446},
+}
+
+
+def measured_traffic(workload: str, form: str, kernel: str):
+    """Measured HBM bytes per launch of `kernel`, or None where none was taken."""
+    return MEASURED_TRAFFIC.get((workload, form), {}).get(kernel)

@@ -110,7 +110,7 @@ def forward(params, norm, X, return_cache=False):
     y = _mm(bf16(W3), h2) + b3[:, None]
     ys = norm[N_YS:N_YS + N_OUT].astype(np.float32)
     ym = norm[N_YM:N_YM + N_OUT].astype(np.float32)
-    rates = (y[:N_OUT].T * ys + ym).astype(np.float32)
+    rates = np.ascontiguousarray((y[:N_OUT].T * ys + ym).astype(np.float32))
     if return_cache:
         return rates, dict(xb=xb, z1=z1, h1=h1, z2=z2, h2=h2, y=y)
     return rates
