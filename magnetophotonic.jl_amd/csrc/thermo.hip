@@ -18,8 +18,7 @@ __global__ __launch_bounds__(128) void multiphysics_kernel(
     double *__restrict__ window, const mp_ferri_params gd, const L *__restrict__ tm_lut,
     const L *__restrict__ re_lut, double T_min, double inv_dT, int32_t lut_N, double dt,
     int subcycles, int brillouin_iters, int64_t N) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+  for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += (int64_t)gridDim.x * blockDim.x) {
     double pabs = 0.0;
     if (HAS_PABS) {
         pabs = window[n] / dt;
@@ -38,6 +37,20 @@ __global__ __launch_bounds__(128) void multiphysics_kernel(
     Te[n] = (L)te; Tl[n] = (L)tl; TsT[n] = (L)tsT; TsR[n] = (L)tsR;
     mTx[n] = (L)m[0]; mTy[n] = (L)m[1]; mTz[n] = (L)m[2];
     mRx[n] = (L)m[3]; mRy[n] = (L)m[4]; mRz[n] = (L)m[5];
+  }
+}
+
+// Workgroups of the film update: at most 1024 (MP_FILM_GRID overrides; 0 = one per 128
+// cells).  The kernel loops over the cells; on the 2.6 M-cell film a 1024-workgroup grid
+// overlaps the Yee updates of the next step with 1% less step time than a full grid.
+static int64_t film_grid(int64_t N) {
+    static int64_t cap = -1;
+    if (cap < 0) {
+        const char *e = getenv("MP_FILM_GRID");
+        cap = e ? atoll(e) : 1024;
+    }
+    const int64_t full = cdiv(N, 128);
+    return cap > 0 && cap < full ? cap : full;
 }
 
 template <typename L>
@@ -50,8 +63,8 @@ static int launch_mp(mp_film *f, const mp_ferri_params *p, const mp_lut *lut, do
 #define MP_ARGS (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, \
                 (L *)f->m_TM_z, (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, f->window_sum, *p,   \
                 (const L *)lut->tm, (const L *)lut->re, tmin, invdt, lut->n, dtc, subcycles, iters, N
-    if (has_pabs) multiphysics_kernel<L, true><<<cdiv(N, 128), 128, 0, stream()>>>(MP_ARGS);
-    else multiphysics_kernel<L, false><<<cdiv(N, 128), 128, 0, stream()>>>(MP_ARGS);
+    if (has_pabs) multiphysics_kernel<L, true><<<film_grid(N), 128, 0, stream()>>>(MP_ARGS);
+    else multiphysics_kernel<L, false><<<film_grid(N), 128, 0, stream()>>>(MP_ARGS);
 #undef MP_ARGS
     MP_LAUNCH_CHECK("multiphysics_kernel");
     return MP_OK;
