@@ -27,7 +27,8 @@ sys.path.insert(0, os.path.join(ROOT, "magnetophotonic.jl_amd"))
 METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Δm|≤1e-3"
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "multiphysics",
-           "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate")
+           "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate", "dft_snapshot", "dft_batch")
+DFT_BATCH = int(os.environ.get("MP_DFT_BATCH", "32"))
 
 
 def build_workload(name: str, form: str):
@@ -152,7 +153,8 @@ def main():
     nx, ny, nz = spec.shape
     ncell = nx * ny * nz
     mcell = ncell * args.steps / wall / 1e6
-    kb = roofline.step_bytes(spec, planes, len(bins))
+    eps_idx = st.yee.n_eps > 0 and os.environ.get("MP_YEE_NO_EPS_INDEX") is None
+    kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH)
     if "multiphysics" in kb:
         kb["surrogate"] = kb["multiphysics"]
     # the dominant kernel is the largest HBM consumer of the step (algorithmic bytes);
@@ -181,10 +183,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": meta["name"], "grid": list(spec.shape), "film_cells": spec.n_film,
                    "form": args.form, "dft_planes": len(planes), "dft_bins": len(bins),
-                   "llb_substeps": spec.subcycles, "cpml": list(spec.cpml.widths)},
+                   "llb_substeps": spec.subcycles, "cpml": list(spec.cpml.widths),
+                   "inv_eps": f"uint8 index, {st.yee.n_eps} triples" if eps_idx else "FP32 volumes",
+                   "dft_batch": DFT_BATCH},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": roofline.measured_traffic(meta["name"], args.form, dom),
+                     "traffic": roofline.measured_traffic(meta["name"], args.form, dom, eps_idx),
                      "alg_bytes_per_launch": int(per_launch)},
         "step_roofline": {"alg_bytes_per_step": kb["total"],
                           "alg_bytes_per_cell": round(kb["total"] / ncell, 2),

@@ -64,6 +64,15 @@ typedef struct mp_yee {
     const void *inv_eps_x, *inv_eps_y, *inv_eps_z;
     const void *inv_d_cell_x, *inv_d_cell_y, *inv_d_cell_z;
     const void *inv_d_dual_x, *inv_d_dual_y, *inv_d_dual_z;
+    /* Optional compact copy of inv_eps for the FP32 E update: eps_index holds one byte
+     * per cell (volume layout) naming an entry of eps_table, n_eps <= 256 entries of 4
+     * FP32 (inv_eps_x, inv_eps_y, inv_eps_z, unused).  The entries are the volumes' own
+     * values, so the update is the same; it reads 1 byte per cell instead of 12.  NULL /
+     * 0 reads the volumes.  The volumes stay required (sources, gyration, other paths). */
+    const uint8_t *eps_index;
+    const float *eps_table;
+    int32_t n_eps;
+    int32_t _pad_eps;
 } mp_yee;
 
 /* CFS-CPML state (src/fdtd/CPML.jl:79-189).  Per-axis coefficient vectors inv_kappa,

@@ -1,0 +1,12 @@
+// Step-driver side of the DFT record (monitors.hip): the batched form records a step into
+// a snapshot slot and adds a full batch to the accumulators; dft_flush adds what is
+// pending.  Bit-identical to recording each step with mp_probe_dft_accumulate.
+#pragma once
+#include "common.h"
+
+namespace mp {
+int dft_batch_size();
+int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes, const double *omega_bins,
+                       double *phase, int32_t nb, double t, double dt);
+int dft_flush(const mp_yee *g, const mp_plane *planes, int32_t nplanes, int32_t nb, double dt);
+}  // namespace mp

@@ -10,12 +10,14 @@
 #include <mutex>
 
 #include "common.h"
+#include "monitors.h"
 
 namespace mp {
 
 struct PlaneDesc {
     int32_t axis;
     int64_t index, n1, n2;
+    int64_t off;  // first cell of this plane in a snapshot slot
     double *ea, *eb, *ha, *hb, *energy;
 };
 struct PlaneSet {
@@ -67,6 +69,119 @@ __global__ __launch_bounds__(256) void dft_accumulate_kernel(
 }
 
 // ---- deterministic reductions -------------------------------------------------------------
+// ---- batched DFT record (used by the step driver) ------------------------------------
+// Each step stores the 4 tangential fields of every plane cell in one slot of a snapshot
+// buffer and the step's phase factors in one slot of a phase buffer; every DFT_BATCH
+// steps (and when the driver returns) one pass adds the stored steps to the FP64
+// accumulators in step order.  The additions are the per-step ones in the same order, so
+// the accumulators are bit-identical to recording every step, with the accumulators read
+// and written once per batch instead of once per step.
+constexpr int DFT_MAXNB = 8;         // bins the batched record supports
+constexpr int DFT_MAXNB_STEPS = 64;  // longest batch
+
+template <typename S> struct Quad { S a, b, c, d; };
+
+template <typename S>
+__global__ __launch_bounds__(256) void dft_snapshot_kernel(PlaneSet ps, const S *Ex, const S *Ey, const S *Ez,
+                                                           const S *Hx, const S *Hy, const S *Hz,
+                                                           Quad<S> *__restrict__ snap, int64_t nx, int64_t ny) {
+    const PlaneDesc &pl = ps.p[blockIdx.y];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = pl.n1 * pl.n2;
+    if (c >= np) return;
+    const int64_t a = c % pl.n1, b = c / pl.n1;
+    const int64_t o = plane_lin(pl.axis, pl.index, a, b, nx, ny);
+    const S *Ea, *Eb, *Ha, *Hb;
+    if (pl.axis == 0) { Ea = Ey; Eb = Ez; Ha = Hy; Hb = Hz; }
+    else if (pl.axis == 1) { Ea = Ez; Eb = Ex; Ha = Hz; Hb = Hx; }
+    else { Ea = Ex; Eb = Ey; Ha = Hx; Hb = Hy; }
+    snap[pl.off + c] = Quad<S>{Ea[o], Eb[o], Ha[o], Hb[o]};
+}
+
+// One phase slot: the nb factors (cos wt * dt, -sin wt * dt) of one step.
+__global__ void phase_slot_kernel(double *phase, const double *omega, int nb, double t, double dt) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nb) return;
+    const double ph = omega[q] * t;
+    phase[2 * q] = cos(ph) * dt;
+    phase[2 * q + 1] = -sin(ph) * dt;
+}
+
+template <typename S, int NB>
+__global__ __launch_bounds__(256) void dft_batch_kernel(PlaneSet ps, const Quad<S> *__restrict__ snap,
+                                                        int64_t slot_stride, const double *__restrict__ phase,
+                                                        int nsteps, double dt) {
+    // the batch's phase factors, read by every cell, staged once per workgroup
+    __shared__ double ph[DFT_MAXNB_STEPS][NB][2];
+    for (int i = threadIdx.x; i < nsteps * NB * 2; i += blockDim.x) {
+        const int st = i / (NB * 2), r = i % (NB * 2);
+        ph[st][r / 2][r % 2] = phase[2 * DFT_MAXNB * st + r];
+    }
+    __syncthreads();
+    const PlaneDesc &pl = ps.p[blockIdx.y];
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = pl.n1 * pl.n2;
+    if (c >= np) return;
+    const Quad<S> *sp = snap + pl.off + c;
+    double acc[4][NB][2];
+    double en = pl.energy[c];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int64_t z = 2 * (c + np * q);
+        const double2 va = *(const double2 *)(pl.ea + z), vb = *(const double2 *)(pl.eb + z);
+        const double2 ha = *(const double2 *)(pl.ha + z), hb = *(const double2 *)(pl.hb + z);
+        acc[0][q][0] = va.x; acc[0][q][1] = va.y;
+        acc[1][q][0] = vb.x; acc[1][q][1] = vb.y;
+        acc[2][q][0] = ha.x; acc[2][q][1] = ha.y;
+        acc[3][q][0] = hb.x; acc[3][q][1] = hb.y;
+    }
+    for (int s0 = 0; s0 < nsteps; s0 += 4) {
+        Quad<S> f[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (s0 + u < nsteps) f[u] = sp[(s0 + u) * slot_stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (s0 + u >= nsteps) break;
+            const double ea = (double)f[u].a, eb = (double)f[u].b, ha = (double)f[u].c, hb = (double)f[u].d;
+            en += (ea * hb - eb * ha) * dt;
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const double pr = ph[s0 + u][q][0], pi = ph[s0 + u][q][1];
+                acc[0][q][0] += ea * pr; acc[0][q][1] += ea * pi;
+                acc[1][q][0] += eb * pr; acc[1][q][1] += eb * pi;
+                acc[2][q][0] += ha * pr; acc[2][q][1] += ha * pi;
+                acc[3][q][0] += hb * pr; acc[3][q][1] += hb * pi;
+            }
+        }
+    }
+    pl.energy[c] = en;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int64_t z = 2 * (c + np * q);
+        *(double2 *)(pl.ea + z) = make_double2(acc[0][q][0], acc[0][q][1]);
+        *(double2 *)(pl.eb + z) = make_double2(acc[1][q][0], acc[1][q][1]);
+        *(double2 *)(pl.ha + z) = make_double2(acc[2][q][0], acc[2][q][1]);
+        *(double2 *)(pl.hb + z) = make_double2(acc[3][q][0], acc[3][q][1]);
+    }
+}
+
+template <typename S>
+static void launch_dft_batch(dim3 grid, const PlaneSet &ps, const void *snap, int64_t total, const double *phase,
+                             int n, int nb, double dt) {
+    const Quad<S> *q = (const Quad<S> *)snap;
+    switch (nb) {
+    case 1: dft_batch_kernel<S, 1><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 2: dft_batch_kernel<S, 2><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 3: dft_batch_kernel<S, 3><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 4: dft_batch_kernel<S, 4><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 5: dft_batch_kernel<S, 5><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 6: dft_batch_kernel<S, 6><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    case 7: dft_batch_kernel<S, 7><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    default: dft_batch_kernel<S, 8><<<grid, 256, 0, stream()>>>(ps, q, total, phase, n, dt); break;
+    }
+}
+
 constexpr int RED_BLOCKS = 256;
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAXV = 10;
@@ -189,31 +304,127 @@ static void plane_fields(const mp_yee *g, int axis, const void **Ea, const void 
 
 using namespace mp;
 
-MP_API int mp_probe_dft_accumulate(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
-                                   const double *omega_bins, double *phase, int32_t nb, double t,
-                                   double dt) {
-    if (!g || (!planes && nplanes > 0)) return fail_arg("mp_probe_dft_accumulate: null argument");
-    if (nplanes <= 0 || nb <= 0) return MP_OK;  // Kernels.jl:677-679
-    if (nplanes > MP_MAX_PLANES) return fail_arg("mp_probe_dft_accumulate: at most %d planes", MP_MAX_PLANES);
-    if (!omega_bins || !phase) return fail_arg("mp_probe_dft_accumulate: null bins/phase buffer");
-    PlaneSet ps{};
+namespace mp {
+
+// The plane set of one record, validated; off = running offset of each plane in a
+// snapshot slot, *total = cells of all planes.
+static int build_planes(const mp_yee *g, const mp_plane *planes, int32_t nplanes, PlaneSet &ps, int64_t *maxnp,
+                        int64_t *total, const char *who) {
+    if (nplanes > MP_MAX_PLANES) return fail_arg("%s: at most %d planes", who, MP_MAX_PLANES);
+    ps = PlaneSet{};
     ps.count = nplanes;
-    int64_t maxnp = 0;
+    *maxnp = 0;
+    *total = 0;
     for (int p = 0; p < nplanes; ++p) {
         const mp_plane &src = planes[p];
-        if (src.axis < 0 || src.axis > 2) return fail_arg("mp_probe_dft_accumulate: bad plane axis");
+        if (src.axis < 0 || src.axis > 2) return fail_arg("%s: bad plane axis", who);
         const int64_t nax = src.axis == 0 ? g->nx : (src.axis == 1 ? g->ny : g->nz);
-        if (src.index < 0 || src.index >= nax) return fail_arg("mp_probe_dft_accumulate: plane index out of range");
+        if (src.index < 0 || src.index >= nax) return fail_arg("%s: plane index out of range", who);
         if (!src.dft_ea || !src.dft_eb || !src.dft_ha || !src.dft_hb || !src.energy)
-            return fail_arg("mp_probe_dft_accumulate: null plane buffer");
+            return fail_arg("%s: null plane buffer", who);
         PlaneDesc &d = ps.p[p];
         d.axis = src.axis;
         d.index = src.index;
         d.n1 = src.axis == 0 ? g->ny : (src.axis == 1 ? g->nz : g->nx);
         d.n2 = src.axis == 0 ? g->nz : (src.axis == 1 ? g->nx : g->ny);
+        d.off = *total;
         d.ea = src.dft_ea; d.eb = src.dft_eb; d.ha = src.dft_ha; d.hb = src.dft_hb; d.energy = src.energy;
-        maxnp = std::max(maxnp, d.n1 * d.n2);
+        *maxnp = std::max(*maxnp, d.n1 * d.n2);
+        *total += d.n1 * d.n2;
     }
+    return MP_OK;
+}
+
+struct DftBatch {
+    void *snap = nullptr;
+    size_t snap_bytes = 0;
+    double *phase = nullptr;
+    int count = 0;
+};
+static DftBatch g_dft;
+
+int dft_batch_size() {
+    static const int k = [] {
+        const char *e = getenv("MP_DFT_BATCH");
+        const int v = e ? atoi(e) : 32;
+        return v > 0 ? v : 1;
+    }();
+    return k;
+}
+
+int dft_flush(const mp_yee *g, const mp_plane *planes, int32_t nplanes, int32_t nb, double dt) {
+    if (g_dft.count == 0) return MP_OK;
+    PlaneSet ps;
+    int64_t maxnp, total;
+    if (int e = build_planes(g, planes, nplanes, ps, &maxnp, &total, "dft flush")) return e;
+    const int n = g_dft.count;
+    g_dft.count = 0;
+    dim3 grid(cdiv(maxnp, 256), nplanes);
+    KernelTimer t("dft_batch");
+    if (g->field_dtype == MP_F32) launch_dft_batch<float>(grid, ps, g_dft.snap, total, g_dft.phase, n, nb, dt);
+    else launch_dft_batch<double>(grid, ps, g_dft.snap, total, g_dft.phase, n, nb, dt);
+    MP_LAUNCH_CHECK("dft_batch_kernel");
+    return MP_OK;
+}
+
+int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes, const double *omega_bins,
+                       double *phase, int32_t nb, double t, double dt) {
+    const int K = dft_batch_size();
+    if (K <= 1 || nb > DFT_MAXNB || (g->field_dtype != MP_F32 && g->field_dtype != MP_F64))
+        return mp_probe_dft_accumulate(g, planes, nplanes, omega_bins, phase, nb, t, dt);
+    if (!g || (!planes && nplanes > 0)) return fail_arg("dft record: null argument");
+    if (nplanes <= 0 || nb <= 0) return MP_OK;
+    if (!omega_bins) return fail_arg("dft record: null bins");
+    PlaneSet ps;
+    int64_t maxnp, total;
+    if (int e = build_planes(g, planes, nplanes, ps, &maxnp, &total, "dft record")) return e;
+    const size_t qsz = g->field_dtype == MP_F32 ? sizeof(Quad<float>) : sizeof(Quad<double>);
+    const size_t need = qsz * (size_t)total * (size_t)K;
+    if (g_dft.snap_bytes < need) {
+        if (g_dft.count) return fail_arg("dft record: plane set changed inside a batch");
+        if (g_dft.snap) MP_HIP(hipFree(g_dft.snap));
+        g_dft.snap = nullptr;
+        g_dft.snap_bytes = 0;
+        MP_HIP(hipMalloc(&g_dft.snap, need));
+        g_dft.snap_bytes = need;
+    }
+    if (K > DFT_MAXNB_STEPS) return fail_arg("dft record: batch longer than %d steps", DFT_MAXNB_STEPS);
+    if (!g_dft.phase) MP_HIP(hipMalloc((void **)&g_dft.phase, sizeof(double) * 2 * DFT_MAXNB * DFT_MAXNB_STEPS));
+    {
+        KernelTimer t0("dft_phase");
+        phase_slot_kernel<<<1, 64, 0, stream()>>>(g_dft.phase + 2 * DFT_MAXNB * g_dft.count, omega_bins, nb, t, dt);
+        MP_LAUNCH_CHECK("phase_slot_kernel");
+    }
+    {
+        dim3 grid(cdiv(maxnp, 256), nplanes);
+        KernelTimer t1("dft_snapshot");
+        if (g->field_dtype == MP_F32)
+            dft_snapshot_kernel<float><<<grid, 256, 0, stream()>>>(
+                ps, (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (const float *)g->hx,
+                (const float *)g->hy, (const float *)g->hz, (Quad<float> *)g_dft.snap + total * g_dft.count, g->nx,
+                g->ny);
+        else
+            dft_snapshot_kernel<double><<<grid, 256, 0, stream()>>>(
+                ps, (const double *)g->ex, (const double *)g->ey, (const double *)g->ez, (const double *)g->hx,
+                (const double *)g->hy, (const double *)g->hz, (Quad<double> *)g_dft.snap + total * g_dft.count,
+                g->nx, g->ny);
+        MP_LAUNCH_CHECK("dft_snapshot_kernel");
+    }
+    if (++g_dft.count == K) return dft_flush(g, planes, nplanes, nb, dt);
+    return MP_OK;
+}
+
+}  // namespace mp
+
+MP_API int mp_probe_dft_accumulate(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
+                                   const double *omega_bins, double *phase, int32_t nb, double t,
+                                   double dt) {
+    if (!g || (!planes && nplanes > 0)) return fail_arg("mp_probe_dft_accumulate: null argument");
+    if (nplanes <= 0 || nb <= 0) return MP_OK;  // Kernels.jl:677-679
+    if (!omega_bins || !phase) return fail_arg("mp_probe_dft_accumulate: null bins/phase buffer");
+    PlaneSet ps;
+    int64_t maxnp, total;
+    if (int e = build_planes(g, planes, nplanes, ps, &maxnp, &total, "mp_probe_dft_accumulate")) return e;
     {
         KernelTimer t0("dft_phase");
         phase_factors_kernel<<<cdiv(nb, 64), 64, 0, stream()>>>(phase, omega_bins, nb, t, dt);

@@ -9,6 +9,7 @@
 #include <cstdlib>
 
 #include "common.h"
+#include "monitors.h"
 
 using namespace mp;
 
@@ -124,8 +125,7 @@ static int one_step(mp_state *s) {
         }
     }
     if (s->record_dft && s->nplanes > 0) {
-        if ((rc = mp_probe_dft_accumulate(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t,
-                                          s->dt)))
+        if ((rc = dft_record_batched(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t, s->dt)))
             return rc;
     }
     return MP_OK;
@@ -138,6 +138,14 @@ MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
     for (int64_t k = 0; k < nsteps; ++k) {
         int rc = one_step(s);
         if (rc) {
+            if (s->record_dft && s->nplanes > 0) dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt);
+            overlap_join();
+            return rc;
+        }
+    }
+    // DFT steps still held in the batch reach the accumulators before the call returns
+    if (s->record_dft && s->nplanes > 0) {
+        if (int rc = dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt)) {
             overlap_join();
             return rc;
         }

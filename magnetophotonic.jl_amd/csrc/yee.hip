@@ -332,15 +332,23 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     }
 }
 
-template <typename P, int TYV>
+// MI: inv_eps comes from the per-cell byte index and a table of triples staged in LDS
+// instead of the three volumes (same values, 1 byte per cell instead of 12).
+template <typename P, int TYV, bool MI>
 __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ Ez,
     const float *__restrict__ Hx, const float *__restrict__ Hy, const float *__restrict__ Hz,
     float *__restrict__ Dx, float *__restrict__ Dy, float *__restrict__ Dz,
     const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
+    const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
     const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, int64_t nby,
     int64_t nbz, int remap) {
+    __shared__ float4 tab[MI ? 256 : 1];
+    if (MI) {
+        for (int t = threadIdx.y * 64 + threadIdx.x; t < nmat; t += 64 * TYV) tab[t] = mtab[t];
+        __syncthreads();
+    }
     const int lane = threadIdx.x;
     const Tile tl = tile_of(nbx, nby, nbz, remap != 0);
     const int64_t i0 = (tl.bx * 64 + lane) * VX;
@@ -368,8 +376,19 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
         const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
         const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
         float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
-        float4 ex, ey, ez;   // fully overwritten except cell i = 0 (stored element-wise below)
-        const float4 iex4 = ld4nt(iex + o), iey4 = ld4nt(iey + o), iez4 = ld4nt(iez + o);
+        float4 iex4, iey4, iez4;
+        if (MI) {
+            const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
+            const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+            const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+            iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+            iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+            iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+        } else {
+            iex4 = ld4nt(iex + o);
+            iey4 = ld4nt(iey + o);
+            iez4 = ld4nt(iez + o);
+        }
         // x neighbours of element 0: element 3 of the previous lane's group
         float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
         if (lane == 0 && ic > 0) {
@@ -415,17 +434,19 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
             el(dx, e) = dxn;
             el(dy, e) = dyn;
             el(dz, e) = dzn;
-            el(ex, e) = dxn * elc(iex4, e);
-            el(ey, e) = dyn * elc(iey4, e);
-            el(ez, e) = dzn * elc(iez4, e);
         }
+        // E = D * inv_eps, formed after the loop so E does not occupy registers through it
         if (act && ic > 0) {
             st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
-            st4nt(Ex + o, ex); st4nt(Ey + o, ey); st4nt(Ez + o, ez);
+            st4nt(Ex + o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
+            st4nt(Ey + o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
+            st4nt(Ez + o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
         } else if (act) {   // the group holding i = 0: that cell is not part of the E update
             for (int e = 1; e < VX; ++e) {
                 Dx[o + e] = elc(dx, e); Dy[o + e] = elc(dy, e); Dz[o + e] = elc(dz, e);
-                Ex[o + e] = elc(ex, e); Ey[o + e] = elc(ey, e); Ez[o + e] = elc(ez, e);
+                Ex[o + e] = elc(dx, e) * elc(iex4, e);
+                Ey[o + e] = elc(dy, e) * elc(iey4, e);
+                Ez[o + e] = elc(dz, e) * elc(iez4, e);
             }
         }
         hx_p = hx_c;
@@ -472,13 +493,15 @@ static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
     return MP_OK;
 }
 
-// Vector-form launch shape: 64 lanes x TYV rows per workgroup, z chunks of VKC planes.
+// Vector-form launch shape: 64 lanes x TYV rows per workgroup, z chunks of MP_YEE_KCHUNK
+// planes (default 1: on the 512 x 512 x 256 step, chunks of 1 / 2 / 4 / 8 planes give an
+// E update of 793 / 817 / 840 / 880 us).
 constexpr int TYV = 4;
 static int64_t vec_kchunk() {
     static int64_t kc = [] {
         const char *e = getenv("MP_YEE_KCHUNK");
-        const long v = e ? atol(e) : 2;
-        return (int64_t)(v > 0 ? v : 2);
+        const long v = e ? atol(e) : 1;
+        return (int64_t)(v > 0 ? v : 1);
     }();
     return kc;
 }
@@ -513,12 +536,18 @@ static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
     const int64_t nbx = cdiv(nx, 64 * VX), nby = cdiv(ny, TYV), nbz = cdiv(nz, kc);
     dim3 block(64, TYV, 1);
     KernelTimer t("yee_E");
-    yee_E_vec_kernel<P, TYV><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
-        (float *)g->ex, (float *)g->ey, (float *)g->ez, (const float *)g->hx, (const float *)g->hy,
-        (const float *)g->hz, (float *)g->dx, (float *)g->dy, (float *)g->dz, (const float *)g->inv_eps_x,
-        (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, (const float *)g->inv_d_dual_x,
-        (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz,
-        kc, (float)dt, nbx, nby, nbz, xcd_remap());
+    const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
+                    getenv("MP_YEE_NO_EPS_INDEX") == nullptr;
+#define E_ARGS                                                                                             \
+    (float *)g->ex, (float *)g->ey, (float *)g->ez, (const float *)g->hx, (const float *)g->hy,            \
+        (const float *)g->hz, (float *)g->dx, (float *)g->dy, (float *)g->dz, (const float *)g->inv_eps_x, \
+        (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, g->eps_index,                            \
+        (const float4 *)g->eps_table, g->n_eps, (const float *)g->inv_d_dual_x,                            \
+        (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz, \
+        kc, (float)dt, nbx, nby, nbz, xcd_remap()
+    if (mi) yee_E_vec_kernel<P, TYV, true><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
+    else yee_E_vec_kernel<P, TYV, false><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
+#undef E_ARGS
     MP_LAUNCH_CHECK("yee_E_vec_kernel");
     return MP_OK;
 }

@@ -22,7 +22,8 @@ class Yee(C.Structure):
                 ("dx", vp), ("dy", vp), ("dz", vp),
                 ("inv_eps_x", vp), ("inv_eps_y", vp), ("inv_eps_z", vp),
                 ("inv_d_cell_x", vp), ("inv_d_cell_y", vp), ("inv_d_cell_z", vp),
-                ("inv_d_dual_x", vp), ("inv_d_dual_y", vp), ("inv_d_dual_z", vp)]
+                ("inv_d_dual_x", vp), ("inv_d_dual_y", vp), ("inv_d_dual_z", vp),
+                ("eps_index", vp), ("eps_table", vp), ("n_eps", i32), ("_pad_eps", i32)]
 
 
 class Cpml(C.Structure):

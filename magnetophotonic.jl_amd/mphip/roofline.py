@@ -29,8 +29,9 @@ def _psi_cells(spec) -> int:
     return tot
 
 
-def kernel_bytes(spec) -> dict:
-    """Algorithmic bytes per launch of each kernel of one step."""
+def kernel_bytes(spec, eps_index: bool = False) -> dict:
+    """Algorithmic bytes per launch of each kernel of one step.  eps_index: the E update
+    reads inv_eps as a 1-byte material index per cell instead of three volumes."""
     nx, ny, nz = spec.shape
     fs = spec.field_dtype.itemsize
     cs = spec.compute_dtype.itemsize
@@ -39,7 +40,7 @@ def kernel_bytes(spec) -> dict:
     out = {}
     # H: read E(3), H(3); write H(3).  E: read H(3), D(3), inv_eps(3); write D(3), E(3).
     out["yee_H"] = 9 * fs * n_upd + 2 * ps * _psi_cells(spec)
-    out["yee_E"] = 15 * fs * n_upd + 2 * ps * _psi_cells(spec)
+    out["yee_E"] = ((12 * fs + 1) if eps_index else 15 * fs) * n_upd + 2 * ps * _psi_cells(spec)
     if spec.ade is not None:
         np_ = len(spec.diag_poles) // 3
         ss = _SZ[spec.ade_state_dtype]
@@ -68,10 +69,25 @@ def plane_bytes(spec, planes, nb: int) -> int:
     return sum(dims[a] * (4 * fs + 4 * nb * 32 + 16) for a, _ in planes)
 
 
-def step_bytes(spec, planes=(), nb: int = 0) -> dict:
-    kb = kernel_bytes(spec)
+def plane_cells(spec, planes) -> int:
+    nx, ny, nz = spec.shape
+    dims = {0: ny * nz, 1: nz * nx, 2: nx * ny}
+    return sum(dims[a] for a, _ in planes)
+
+
+def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch: int = 1) -> dict:
+    """Algorithmic bytes per step.  With dft_batch > 1 the DFT record is a per-step
+    snapshot (4 fields read, 4 written) plus, once per batch, the stored steps read and
+    the accumulators read and written once."""
+    kb = kernel_bytes(spec, eps_index)
     if planes and nb:
-        kb["dft_accumulate"] = plane_bytes(spec, planes, nb)
+        if dft_batch > 1:
+            fs = spec.field_dtype.itemsize
+            cells = plane_cells(spec, planes)
+            kb["dft_snapshot"] = cells * 8 * fs
+            kb["dft_batch"] = cells * (4 * fs + 2 * (4 * nb * 16 + 8) // dft_batch)
+        else:
+            kb["dft_accumulate"] = plane_bytes(spec, planes, nb)
     kb["total"] = int(sum(v for v in kb.values()))
     return kb
 
@@ -80,10 +96,11 @@ def step_bytes(spec, planes=(), nb: int = 0) -> dict:
 # KiB per dispatch; the factors are what the counters read for a plain copy of known size
 # on gfx950), in runs separate from the timed one: profiles/yee_E_coupled_pump_512x512x256_pmc.md.
 MEASURED_TRAFFIC = {
-    ("coupled_pump_512x512x256", "plain"): {"yee_E": 4349117132, "yee_H": 2762425446},
+    ("coupled_pump_512x512x256", "plain", "volumes"): {"yee_E": 4349117132, "yee_H": 2762425446},
 }
 
 
-def measured_traffic(workload: str, form: str, kernel: str):
+def measured_traffic(workload: str, form: str, kernel: str, eps_index: bool = False):
     """Measured HBM bytes per launch of `kernel`, or None where none was taken."""
-    return MEASURED_TRAFFIC.get((workload, form), {}).get(kernel)
+    key = (workload, form, "index" if eps_index else "volumes")
+    return MEASURED_TRAFFIC.get(key, {}).get(kernel)

@@ -209,3 +209,43 @@ def build_state_spec(grid: Grid3D, geo: Geometry, dt: float, params: FDTDParams 
                      multiphysics_every=int(multiphysics_every), subcycles=int(subs),
                      brillouin_iters=int(brillouin_iters), multiphysics_enabled=bool(enable_multiphysics),
                      absorption_model=absorption_model, has_film=has_film)
+
+
+def _small_codes(a: np.ndarray, limit: int):
+    """Sorted distinct values of `a` and each element's position among them, or None when
+    there are more than `limit` (a subsample proposes candidates; every element is then
+    checked against them, so the result is exact)."""
+    flat = np.ascontiguousarray(a).reshape(-1)
+    cand = np.unique(flat[:: max(1, flat.size // 65536)])
+    for _ in range(4):
+        if cand.size > limit:
+            return None
+        pos = np.minimum(np.searchsorted(cand, flat), cand.size - 1)
+        miss = cand[pos] != flat
+        if not miss.any():
+            return cand, pos
+        cand = np.union1d(cand, np.unique(flat[miss]))
+    return None
+
+
+def eps_index(inv_eps, limit: int = 256):
+    """Compact form of the three inv_eps volumes for the FP32 E update: a uint8 index
+    per cell and a (n, 4) FP32 table of the distinct (x, y, z) triples, or None when
+    there are more than `limit` distinct triples.  The table holds the volumes' own
+    values, so the update that reads it is unchanged."""
+    if any(np.asarray(a).dtype != np.float32 for a in inv_eps):
+        return None
+    codes = [_small_codes(a, limit) for a in inv_eps]
+    if any(c is None for c in codes):
+        return None
+    (vx, px), (vy, py), (vz, pz) = codes
+    key = (px.astype(np.int64) * vy.size + py) * vz.size + pz
+    kc = _small_codes(key, limit)
+    if kc is None:
+        return None
+    keys, index = kc
+    table = np.zeros((keys.size, 4), np.float32)
+    table[:, 0] = vx[keys // (vy.size * vz.size)]
+    table[:, 1] = vy[(keys // vz.size) % vy.size]
+    table[:, 2] = vz[keys % vz.size]
+    return index.astype(np.uint8), table

@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib, abi, dispersion
 from .sources import (AXIS_CODE, COMPONENT_CODE, CarrierGaussianPulse, ContinuousSource, GaussianPulse,
                       ModeSource, PlaneSource, SoftSource, axis_weights, source_index)
-from .spec import StateSpec
+from .spec import StateSpec, eps_index
 
 _DT_CODE = {np.dtype(np.float32): abi.MP_F32, np.dtype(np.float64): abi.MP_F64}
 _PSI_CODE = {"f32": abi.MP_F32, "f64": abi.MP_F64, "bf16": abi.MP_BF16, "f16": abi.MP_F16}
@@ -119,6 +119,10 @@ class FDTDState:
         y.inv_eps_x, y.inv_eps_y, y.inv_eps_z = (b.ptr for b in self.inv_eps)
         y.inv_d_cell_x, y.inv_d_cell_y, y.inv_d_cell_z = (b.ptr for b in self.idc)
         y.inv_d_dual_x, y.inv_d_dual_y, y.inv_d_dual_z = (b.ptr for b in self.idd)
+        ei = eps_index(spec.inv_eps) if FT == np.float32 and CT == np.float32 else None
+        if ei is not None:
+            self._eps_index, self._eps_table = dev_arr(ei[0]), dev_arr(ei[1])
+            y.eps_index, y.eps_table, y.n_eps = self._eps_index.ptr, self._eps_table.ptr, ei[1].shape[0]
         self.yee = y
 
         # CPML

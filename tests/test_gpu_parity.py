@@ -270,3 +270,31 @@ def test_coupled_pump_batched_run_overlap():
         for k in ("ea", "eb", "ha", "hb", "energy"):
             assert rel_l2(g[k], r[k]) < 1e-5, (p, k)
     gpu.free()
+
+
+def test_batched_dft_record_is_bitwise_per_step():
+    """Inside one mp_run_steps call the DFT record is batched (snapshots, then one pass over
+    the stored steps in step order); one step per call records step by step.  The two must
+    give the same bits."""
+    from mphip import configs
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+    spec, meta = configs.coupled_pump(nx=40, ny=36, nz=64, steps=200, fluence_mj_cm2=12.0, npml=6, src_z=40e-9)
+    print("case batched DFT record vs per step 40x36x64", flush=True)
+    a, b = FDTDState(spec), FDTDState(spec)
+    ma = DFTPlanes(a, meta["planes"], meta["omega_bins"])
+    mb = DFTPlanes(b, meta["planes"], meta["omega_bins"])
+    a.attach_dft(ma)
+    b.attach_dft(mb)
+    a.run(37)
+    a.run(40)
+    for _ in range(77):
+        b.step()
+    for p in range(len(meta["planes"])):
+        ga, gb = ma.download(p), mb.download(p)
+        for k in ("ea", "eb", "ha", "hb", "energy"):
+            assert np.array_equal(ga[k], gb[k]), (p, k)
+    for name in FIELDS:
+        assert np.array_equal(a.field(name), b.field(name)), name
+    a.free()
+    b.free()

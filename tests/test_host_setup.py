@@ -84,3 +84,21 @@ def test_em_stencil_config_small():
     assert spec.ade is not None and not spec.has_film
     assert meta["n_ade"] == len(spec.ade[0].idx) > 0
     assert spec.inv_eps[0].min() < spec.inv_eps[0].max()
+
+
+def test_eps_index_reproduces_volumes():
+    """The compact inv_eps form (uint8 index + triple table) gives back every cell's three
+    values exactly, on the pump geometry and on the random-box EM geometry; more than 256
+    distinct triples falls back to the volumes."""
+    import numpy as np
+    from mphip import configs
+    from mphip.spec import eps_index
+    for spec in (configs.coupled_pump(nx=64, ny=48, nz=64, steps=10)[0],
+                 configs.em_stencil(n=64, npml=8, steps=10)[0]):
+        idx, tab = eps_index(spec.inv_eps)
+        assert idx.dtype == np.uint8 and tab.shape[0] <= 256
+        for c in range(3):
+            assert np.array_equal(tab[idx, c], spec.inv_eps[c])
+    rng = np.random.default_rng(0)
+    many = [rng.random(4096).astype(np.float32) for _ in range(3)]
+    assert eps_index(many) is None
