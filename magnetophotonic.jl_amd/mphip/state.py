@@ -439,6 +439,11 @@ class FDTDState:
     def synchronize(self):
         _lib.call("synchronize")
 
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        """Overwrite one field volume from a host (nx, ny, nz) array (adapt_backend)."""
+        a = np.asarray(values, dtype=self.spec.field_dtype)
+        self.f[name.lower()].upload(a.reshape(-1, order="F") if a.ndim == 3 else a)
+
     def field(self, name: str) -> np.ndarray:
         """One field volume as a host (nx, ny, nz) array (to_host)."""
         nx, ny, nz = self.spec.shape

@@ -276,6 +276,10 @@ class OracleState:
             self.step()
         return self
 
+    def set_field(self, name, values):
+        a = np.asarray(values, dtype=self.f[name.lower()].dtype)
+        self.f[name.lower()][...] = a.reshape(-1, order="F") if a.ndim == 3 else a
+
     def field(self, name):
         nx, ny, nz = self.spec.shape
         return self.f[name.lower()].reshape((nx, ny, nz), order="F").copy()
