@@ -98,7 +98,7 @@ def main():
     ap.add_argument("--form", default="plain", choices=("plain", "mixed", "surrogate"))
     ap.add_argument("--tol", type=float, default=1e-4, help="surrogate check tolerance")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB, the known-size "
                          "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters")
