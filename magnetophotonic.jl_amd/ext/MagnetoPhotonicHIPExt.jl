@@ -117,6 +117,7 @@ struct MpYee
     inv_eps_x::Ptr{Cvoid}; inv_eps_y::Ptr{Cvoid}; inv_eps_z::Ptr{Cvoid}
     inv_d_cell_x::Ptr{Cvoid}; inv_d_cell_y::Ptr{Cvoid}; inv_d_cell_z::Ptr{Cvoid}
     inv_d_dual_x::Ptr{Cvoid}; inv_d_dual_y::Ptr{Cvoid}; inv_d_dual_z::Ptr{Cvoid}
+    eps_index::Ptr{UInt8}; eps_table::Ptr{Cfloat}; n_eps::Int32; _pad_eps::Int32
 end
 
 struct MpCpml
@@ -155,6 +156,10 @@ struct MpLut
     n::Int32; dtype::Int32
 end
 
+struct MpMlp
+    params::Ptr{Cfloat}; packed::Ptr{Cvoid}; norm::Ptr{Cfloat}
+end
+
 struct MpPlane
     axis::Int32; _pad::Int32; index::Int64
     dft_ea::Ptr{Cvoid}; dft_eb::Ptr{Cvoid}; dft_ha::Ptr{Cvoid}; dft_hb::Ptr{Cvoid}
@@ -168,7 +173,8 @@ function _yee(f::FieldState{T}, ie_x, ie_y, ie_z, dcx, dcy, dcz, ddx, ddy, ddz, 
     nx, ny, nz = size(f.Ex)
     return MpYee(nx, ny, nz, _dtype(T), _dtype(CT),
                  _p(f.Ex), _p(f.Ey), _p(f.Ez), _p(f.Hx), _p(f.Hy), _p(f.Hz), _p(f.Dx), _p(f.Dy), _p(f.Dz),
-                 _p(ie_x), _p(ie_y), _p(ie_z), _p(dcx), _p(dcy), _p(dcz), _p(ddx), _p(ddy), _p(ddz))
+                 _p(ie_x), _p(ie_y), _p(ie_z), _p(dcx), _p(dcy), _p(dcz), _p(ddx), _p(ddy), _p(ddz),
+                 Ptr{UInt8}(C_NULL), Ptr{Cfloat}(C_NULL), Int32(0), Int32(0))
 end
 
 _psi_dtype(c) = _dtype(eltype(c.psi_Hxy_lo))
@@ -308,12 +314,38 @@ function MagnetoPhotonic._ka_multiphysics_step!(::HIPBackend, thermal::ThermalSt
     gd = Ref(FerrimagnetParameters{Float64}(ntuple(i -> Float64(getfield(model.params, i)),
                                                    fieldcount(typeof(model.params)))...))
     l = Ref(MpLut(_p(tm_lut), _p(re_lut), Float64(T_min), Float64(inv_dT), Int32(lut_N), _dtype(CT)))
-    _check(ccall((:mp_multiphysics_step, libmphip), Cint,
-                 (Ref{MpFilm}, Ref{FerrimagnetParameters{Float64}}, Ref{MpLut}, Cdouble, Int32, Int32, Int32),
-                 f, gd, l, Float64(dt), Int32(subcycles), Int32(brillouin_iters), Int32(absorption !== nothing)),
-           "mp_multiphysics_step")
+    sur = SURROGATE[]
+    if sur === nothing
+        _check(ccall((:mp_multiphysics_step, libmphip), Cint,
+                     (Ref{MpFilm}, Ref{FerrimagnetParameters{Float64}}, Ref{MpLut}, Cdouble, Int32, Int32, Int32),
+                     f, gd, l, Float64(dt), Int32(subcycles), Int32(brillouin_iters), Int32(absorption !== nothing)),
+               "mp_multiphysics_step")
+    else
+        mlp, tol, fallbacks = sur
+        _check(ccall((:mp_surrogate_step, libmphip), Cint,
+                     (Ref{MpFilm}, Ref{FerrimagnetParameters{Float64}}, Ref{MpLut}, Ref{MpMlp}, Cdouble, Int32, Int32,
+                      Int32, Cdouble, Ptr{Int64}),
+                     f, gd, l, Ref(mlp), Float64(dt), Int32(subcycles), Int32(brillouin_iters),
+                     Int32(absorption !== nothing), tol, fallbacks),
+               "mp_surrogate_step")
+    end
     return thermal
 end
+
+# Optional learned film update (no counterpart in the reference): after
+# use_surrogate!(params, packed, norm; tol) every film update of this backend goes through
+# mp_surrogate_step, with the plain integrator as its checked fallback.  params (Float32,
+# MP_MLP_NPARAM), packed (MP_MLP_PACKED_BYTES, filled by mp_mlp_pack) and norm (Float32,
+# MP_MLP_NNORM) are device arrays; fallbacks, if given, is a device Int64 counter.
+const SURROGATE = Ref{Any}(nothing)
+function use_surrogate!(params::HIPArray, packed::HIPArray, norm::HIPArray; tol::Real=1e-4,
+                        fallbacks::Union{Nothing,HIPArray}=nothing)
+    mlp = MpMlp(Ptr{Cfloat}(params.ptr), Ptr{Cvoid}(packed.ptr), Ptr{Cfloat}(norm.ptr))
+    _check(ccall((:mp_mlp_pack, libmphip), Cint, (Ref{MpMlp},), Ref(mlp)), "mp_mlp_pack")
+    SURROGATE[] = (mlp, Float64(tol), fallbacks === nothing ? Ptr{Int64}(C_NULL) : Ptr{Int64}(fallbacks.ptr))
+    return nothing
+end
+clear_surrogate!() = (SURROGATE[] = nothing; nothing)
 
 function MagnetoPhotonic._ka_probe_dft_accumulate!(::HIPBackend, fields,
                                                    dft_Ey_pre, dft_Ez_pre, dft_Hy_pre, dft_Hz_pre,
