@@ -231,16 +231,25 @@ __device__ __forceinline__ void st4nt(float *p, float4 v) {
 // its own L2; remapping the dispatch index gives every XCD a contiguous range of tiles,
 // so a tile and its y neighbour (which shares a halo row) usually meet in the same L2.
 constexpr int NXCD = 8;
-struct Tile { int64_t bx, by, bz; };
-__device__ __forceinline__ Tile tile_of(int64_t nbx, int64_t nby, int64_t nbz, bool remap) {
-    const int64_t nb = nbx * nby * nbz;
+// Flattened plane mapping of the vector kernels: consecutive rows of a z-plane are
+// adjacent in memory, so a plane is one run of nx*ny/4 float4 groups, row after row.
+// Lanes take consecutive groups across row ends (no idle lanes when a row is not a
+// multiple of 256 cells) and every load of a wave stays contiguous.  A lane's x
+// neighbour from a shuffle may belong to the next row; it is needed only for the cell
+// the update skips there (H: i = nx-1, E: i = 0).
+struct PlaneGroup { int64_t i0, j, bz; bool in_plane; };
+__device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_t bpp, int64_t nbz, bool remap,
+                                                  int tyv) {
+    const int64_t nb = bpp * nbz;
     const int64_t b = blockIdx.x;
-    int64_t t = b;
-    if (remap && nb % NXCD == 0) t = (b % NXCD) * (nb / NXCD) + b / NXCD;
-    Tile r;
-    r.bx = t % nbx;
-    r.by = (t / nbx) % nby;
-    r.bz = t / (nbx * nby);
+    const int64_t t = (remap && nb % NXCD == 0) ? (b % NXCD) * (nb / NXCD) + b / NXCD : b;
+    const int64_t G = nx / VX;
+    const int64_t g = (t % bpp) * (64 * tyv) + threadIdx.y * 64 + threadIdx.x;
+    PlaneGroup r;
+    r.bz = t / bpp;
+    r.j = g / G;
+    r.i0 = (g % G) * VX;
+    r.in_plane = g < G * ny;
     return r;
 }
 __device__ __forceinline__ float &el(float4 &v, int e) { return (&v.x)[e]; }
@@ -254,15 +263,13 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt_mu0, int64_t nbx,
     int64_t nby, int64_t nbz, int remap) {
     const int lane = threadIdx.x;
-    const Tile tl = tile_of(nbx, nby, nbz, remap != 0);
-    const int64_t i0 = (tl.bx * 64 + lane) * VX;
-    const int64_t j = tl.by * TYV + threadIdx.y;
+    const PlaneGroup pg = plane_group(nx, ny, nbx, nbz, remap != 0, TYV);
+    const int64_t i0 = pg.i0, j = pg.j;
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
-    const bool row_ok = j < ny - 1;
-    const bool act = row_ok && i0 < nx;
-    const int64_t k0 = tl.bz * kchunk;
+    const bool act = pg.in_plane && j < ny - 1;
+    const int64_t k0 = pg.bz * kchunk;
     const int64_t k1 = min(k0 + kchunk, nz - 1);
-    if (!row_ok || k0 >= k1) return;   // uniform across the wave (same j, same k range)
+    if (k0 >= k1) return;   // uniform across the workgroup
     const int64_t sy = nx, sz = nx * ny;
     const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
     const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
@@ -270,12 +277,13 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     P **ps = (P **)cp.psi;
     const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
     const int64_t ic = act ? i0 : 0;   // clamp addresses of idle lanes
+    const int64_t jc = act ? j : (int64_t)0;   // rows 0 and 1 exist (ny >= 2)
     int64_t jsy;
-    const int ry = pml_slab(j, ny, wy, &jsy);
+    const int ry = pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(idcx + ic), xk4 = ld4(xk + ic);
-    const float idy_ = idcy[j], yk_ = yk[j], ya_ = ya[j], yb_ = yb[j];
+    const float idy_ = idcy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
     const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
-    int64_t o = ic + sy * j + sz * k0;
+    int64_t o = ic + sy * jc + sz * k0;
     float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o);
     for (int64_t k = k0; k < k1; ++k, o += sz) {
         const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
@@ -309,9 +317,9 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
             int64_t jsx = 0;
             const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
             const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-            const int64_t ox = jsx + wx * (j + ny * k);
+            const int64_t ox = jsx + wx * (jc + ny * k);
             const int64_t oy = i + nx * (jsy + wy * k);
-            const int64_t oz = i + nx * (j + ny * jsz);
+            const int64_t oz = i + nx * (jc + ny * jsz);
             const float pHxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
             const float pHxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
             el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz));
@@ -350,14 +358,12 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
         __syncthreads();
     }
     const int lane = threadIdx.x;
-    const Tile tl = tile_of(nbx, nby, nbz, remap != 0);
-    const int64_t i0 = (tl.bx * 64 + lane) * VX;
-    const int64_t j = tl.by * TYV + threadIdx.y;
-    const bool row_ok = j >= 1 && j < ny;
-    const bool act = row_ok && i0 < nx;
-    const int64_t k0 = max(tl.bz * kchunk, (int64_t)1);
-    const int64_t k1 = min(tl.bz * kchunk + kchunk, nz);
-    if (!row_ok || k0 >= k1) return;
+    const PlaneGroup pg = plane_group(nx, ny, nbx, nbz, remap != 0, TYV);
+    const int64_t i0 = pg.i0, j = pg.j;
+    const bool act = pg.in_plane && j >= 1;
+    const int64_t k0 = max(pg.bz * kchunk, (int64_t)1);
+    const int64_t k1 = min(pg.bz * kchunk + kchunk, nz);
+    if (k0 >= k1) return;   // uniform across the workgroup
     const int64_t sy = nx, sz = nx * ny;
     const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
     const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
@@ -365,12 +371,13 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     P **ps = (P **)cp.psi;
     const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
     const int64_t ic = act ? i0 : 0;
+    const int64_t jc = act ? j : (int64_t)1;   // rows 0 and 1 exist (ny >= 2)
     int64_t jsy;
-    const int ry = pml_slab(j, ny, wy, &jsy);
+    const int ry = pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(iddx + ic), xk4 = ld4(xk + ic);
-    const float idy_ = iddy[j], yk_ = yk[j], ya_ = ya[j], yb_ = yb[j];
+    const float idy_ = iddy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
     const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
-    int64_t o = ic + sy * j + sz * k0;
+    int64_t o = ic + sy * jc + sz * k0;
     float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
     for (int64_t k = k0; k < k1; ++k, o += sz) {
         const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
@@ -416,9 +423,9 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
             int64_t jsx = 0;
             const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
             const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-            const int64_t ox = jsx + wx * (j + ny * k);
+            const int64_t ox = jsx + wx * (jc + ny * k);
             const int64_t oy = i + nx * (jsy + wy * k);
-            const int64_t oz = i + nx * (j + ny * jsz);
+            const int64_t oz = i + nx * (jc + ny * jsz);
             const float pDxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
             const float pDxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
             const float cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
@@ -515,8 +522,10 @@ static int xcd_remap() {
 template <typename P>
 static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:60-101)
     const int64_t kc = vec_kchunk();
-    const int64_t nbx = cdiv(nx, 64 * VX), nby = cdiv(ny, TYV), nbz = cdiv(nz, kc);
+    // nbx = workgroups per z-plane of the flattened group mapping; nby is unused (1)
+    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TYV), nby = 1, nbz = cdiv(nz, kc);
     dim3 block(64, TYV, 1);
     const float dt_mu0 = (float)dt / (float)mu0;
     KernelTimer t("yee_H");
@@ -532,8 +541,9 @@ static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0
 template <typename P>
 static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+    if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:103-152)
     const int64_t kc = vec_kchunk();
-    const int64_t nbx = cdiv(nx, 64 * VX), nby = cdiv(ny, TYV), nbz = cdiv(nz, kc);
+    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TYV), nby = 1, nbz = cdiv(nz, kc);
     dim3 block(64, TYV, 1);
     KernelTimer t("yee_E");
     const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
