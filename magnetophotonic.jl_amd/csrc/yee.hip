@@ -519,17 +519,27 @@ static int xcd_remap() {
     return on;
 }
 
-template <typename P>
-static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
+// Rows of 64 lanes per workgroup (MP_YEE_TYV: 2, 4 or 8; default 4).
+static int yee_tyv() {
+    static const int t = [] {
+        const char *e = getenv("MP_YEE_TYV");
+        const int v = e ? atoi(e) : 4;
+        return (v == 2 || v == 8) ? v : 4;
+    }();
+    return t;
+}
+
+template <typename P, int TY>
+static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:60-101)
     const int64_t kc = vec_kchunk();
     // nbx = workgroups per z-plane of the flattened group mapping; nby is unused (1)
-    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TYV), nby = 1, nbz = cdiv(nz, kc);
-    dim3 block(64, TYV, 1);
+    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nby = 1, nbz = cdiv(nz, kc);
+    dim3 block(64, TY, 1);
     const float dt_mu0 = (float)dt / (float)mu0;
     KernelTimer t("yee_H");
-    yee_H_vec_kernel<P, TYV><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
+    yee_H_vec_kernel<P, TY><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
         (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
         (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, kc, dt_mu0, nbx, nby, nbz,
@@ -538,13 +548,13 @@ static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0
     return MP_OK;
 }
 
-template <typename P>
-static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
+template <typename P, int TY>
+static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:103-152)
     const int64_t kc = vec_kchunk();
-    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TYV), nby = 1, nbz = cdiv(nz, kc);
-    dim3 block(64, TYV, 1);
+    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nby = 1, nbz = cdiv(nz, kc);
+    dim3 block(64, TY, 1);
     KernelTimer t("yee_E");
     const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
                     getenv("MP_YEE_NO_EPS_INDEX") == nullptr;
@@ -555,11 +565,29 @@ static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
         (const float4 *)g->eps_table, g->n_eps, (const float *)g->inv_d_dual_x,                            \
         (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz, \
         kc, (float)dt, nbx, nby, nbz, xcd_remap()
-    if (mi) yee_E_vec_kernel<P, TYV, true><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
-    else yee_E_vec_kernel<P, TYV, false><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
+    if (mi) yee_E_vec_kernel<P, TY, true><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
+    else yee_E_vec_kernel<P, TY, false><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
 #undef E_ARGS
     MP_LAUNCH_CHECK("yee_E_vec_kernel");
     return MP_OK;
+}
+
+template <typename P>
+static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
+    switch (yee_tyv()) {
+    case 2: return launch_H_vec_t<P, 2>(g, c, dt, mu0);
+    case 8: return launch_H_vec_t<P, 8>(g, c, dt, mu0);
+    default: return launch_H_vec_t<P, 4>(g, c, dt, mu0);
+    }
+}
+
+template <typename P>
+static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
+    switch (yee_tyv()) {
+    case 2: return launch_E_vec_t<P, 2>(g, c, dt);
+    case 8: return launch_E_vec_t<P, 8>(g, c, dt);
+    default: return launch_E_vec_t<P, 4>(g, c, dt);
+    }
 }
 
 // The vector form needs x rows of a multiple of 4 cells (16-byte aligned rows).
