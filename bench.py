@@ -68,6 +68,24 @@ def train_pump_surrogate(steps: int, seed: int = 1):
                      train_s=round(time.perf_counter() - t0, 1))
 
 
+def relax_accuracy():
+    """Surrogate against plain on the relax phase of the 128 x 128 x 64 accuracy case
+    (held-out 60 mJ/cm^2; network trained on the GPU on rows from 35, 50, 70, 85 mJ/cm^2):
+    film-mean |dm| against the same path run in FP64 on the GPU, relax wall time and
+    fallback fraction.  Not part of the timed region."""
+    from mphip import surrogate as S
+    t0 = time.perf_counter()
+    X, T = S.relax_rows((35.0, 50.0, 70.0, 85.0))
+    rsur = S.train_relax_surrogate(X, T)
+    ref = S.pump_relax_film("f64")
+    res = S.relax_comparison(rsur, ref, tols=(1e-3, 1e-4), plain_subcycles=(10, 2, 3))
+    rsur.free()
+    res["rows"] = int(X.shape[0])
+    res["reference"] = "same path in FP64 on the GPU"
+    res["study_s"] = round(time.perf_counter() - t0, 1)
+    return res
+
+
 def cpu_baseline(spec, meta, steps: int, warmup: int = 1):
     """The C oracle (a restatement of the reference CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -97,6 +115,7 @@ def main():
     ap.add_argument("--workload", default="coupled_pump")
     ap.add_argument("--form", default="plain", choices=("plain", "mixed", "surrogate"))
     ap.add_argument("--tol", type=float, default=1e-4, help="surrogate check tolerance")
+    ap.add_argument("--no-accuracy", action="store_true", help="surrogate form: skip the relax accuracy study")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -198,6 +217,8 @@ def main():
     if sur_info is not None:
         sur_info["tol"] = args.tol
         sur_info["fallback_frac"] = round(sur.fallback_count() / (spec.n_film * args.steps), 5)
+        if not args.no_accuracy:
+            sur_info["relax_accuracy"] = relax_accuracy()
         out["config"]["surrogate"] = sur_info
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, meta, args.cpu_steps)

@@ -237,3 +237,102 @@ def collect_trajectory(st: FDTDState, pump_steps: int, every: int, relax_steps: 
 def shuffled(X: np.ndarray, T: np.ndarray, seed: int = 0):
     perm = np.random.default_rng(seed).permutation(X.shape[0])
     return X[perm], T[perm]
+
+
+# ---- the pump + relax accuracy study ---------------------------------------------------------
+# A 128 x 128 x 64 copy of the coupled pump geometry (film 10 cells of 2 nm, source 60 nm above
+# it, CPML 8): 1500 pump steps at the given fluence, then relax steps of 5 fs with 10 plain
+# substeps.  At 60 mJ/cm^2 about 5 % of the film switches, so the film-mean magnetisation after
+# the relax phase is sensitive to small errors in the film update.
+ACCURACY_CASE = dict(nx=128, ny=128, nz=64, npml=8, src_z=60e-9, n_planes=0)
+FILM_KEYS = ("Te", "Tl", "Ts_TM", "Ts_RE", "m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z")
+
+
+def relax_rows(fluences, pump_steps: int = 1500, relax_steps: int = 600, relax_dt: float = 5e-15,
+               relax_subcycles: int = 10, every: int = 5, stride: int = 4):
+    """Training rows of the relax phase (no heating, relax_dt, plain integrator) after the
+    accuracy case's pump at each fluence: every `every`-th relax step, every `stride`-th cell."""
+    from . import configs
+    Xs, Ts = [], []
+    for fl in fluences:
+        spec, _ = configs.coupled_pump(dtype="f32", steps=pump_steps, fluence_mj_cm2=float(fl), **ACCURACY_CASE)
+        st = FDTDState(spec)
+        st.run(pump_steps)
+        st.flush_multiphysics()
+        for r in range(relax_steps):
+            if r % every == 0:
+                X, T = film_rows(st, relax_dt, relax_subcycles, False, stride)
+                Xs.append(X)
+                Ts.append(T)
+            st.relax_step(relax_dt, subcycles=relax_subcycles)
+        st.free()
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def train_relax_surrogate(X, T, steps: int = 30000, seed: int = 1) -> "Surrogate":
+    X, T = shuffled(X, T, 0)
+    sur = Surrogate(norm=make_norm(X[:200000], T[:200000]), seed=seed)
+    data = Dataset(X, T)
+    sur.train(data, steps, batch=65536, lr=3e-3, lr_end=1e-5)
+    data.free()
+    return sur
+
+
+def pump_relax_film(dtype: str = "f32", fluence: float = 60.0, pump_steps: int = 1500,
+                    relax_steps: int = 600, relax_dt: float = 5e-15, relax_subcycles: int = 10) -> dict:
+    """Film state of the accuracy case after pump + relax, with the plain integrator on the
+    GPU (dtype 'f64' gives the double-precision reference of the same path)."""
+    from . import configs
+    spec, _ = configs.coupled_pump(dtype=dtype, steps=pump_steps, fluence_mj_cm2=fluence, **ACCURACY_CASE)
+    st = FDTDState(spec)
+    st.run(pump_steps)
+    st.flush_multiphysics()
+    for _ in range(relax_steps):
+        st.relax_step(relax_dt, subcycles=relax_subcycles)
+    film = st.film_state()
+    st.free()
+    return film
+
+
+def relax_comparison(sur: "Surrogate", ref_film: dict, fluence: float = 60.0, pump_steps: int = 1500,
+                     relax_steps: int = 600, relax_dt: float = 5e-15, relax_subcycles: int = 10,
+                     tols=(1e-3, 1e-4), plain_subcycles=(10, 1, 2, 3)) -> dict:
+    """Relax phase of the accuracy case from one FP32 pump state, by the plain integrator
+    with several substep counts and by the surrogate at several tolerances: film-mean
+    |dm| against ref_film (max over both sublattices), wall time and fallback fraction."""
+    import time
+
+    from . import _lib, configs
+    spec, _ = configs.coupled_pump(dtype="f32", steps=pump_steps, fluence_mj_cm2=fluence, **ACCURACY_CASE)
+    st = FDTDState(spec)
+    st.run(pump_steps)
+    st.flush_multiphysics()
+    start = st.film_state()
+    n = spec.n_film
+
+    def one(sub, tol=None):
+        for k in FILM_KEYS:
+            st.set_film(k, start[k])
+        if tol is not None:
+            st.attach_surrogate(sur, tol)
+            sur.reset_fallbacks()
+        _lib.call("synchronize")
+        t0 = time.perf_counter()
+        for _ in range(relax_steps):
+            st.relax_step(relax_dt, subcycles=sub)
+        _lib.call("synchronize")
+        ms = 1e3 * (time.perf_counter() - t0)
+        f = st.film_state()
+        fb = sur.fallback_count() / (n * relax_steps) if tol is not None else 0.0
+        st.detach_surrogate()
+        err = max(abs(float(f[k].astype(np.float64).mean() - np.asarray(ref_film[k], np.float64).mean()))
+                  for k in ("m_TM_x", "m_RE_x"))
+        return dict(film_mean_dm=err, relax_ms=round(ms, 2), fallback=round(fb, 4),
+                    switched=float(np.mean(f["m_TM_x"] < 0)))
+
+    out = {f"plain_{s}sub": one(s) for s in plain_subcycles}
+    for tol in tols:
+        out[f"surrogate_tol{tol:g}"] = one(relax_subcycles, tol)
+    out["switched_ref"] = float(np.mean(np.asarray(ref_film["m_TM_x"]) < 0))
+    st.free()
+    return out

@@ -5,7 +5,7 @@ integrator, so fields, raw psi slabs and raw ADE state must equal the oracle's r
 of the same storage form bit for bit.  With the gyration on, the bar is the coupled
 path's: fields within 1e-5 relative L2 and magnetisation within 1e-4.
 
-Accuracy: the coupled pump on a 96 x 96 x 64 copy of the pump geometry (60 mJ/cm^2, 1500
+Accuracy: the coupled pump on a 128 x 128 x 64 copy of the pump geometry (60 mJ/cm^2, 1500
 steps, then 600 relax steps of 5 fs) switches about 5 % of the film.  Each storage form is
 compared with the FP64 oracle on identical inputs.  Plain FP32 meets the film bar (mean
 |dm| <= 1e-3); binary16 storage stays within a few per cent on fields and temperature but
@@ -17,7 +17,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("ex", "ey", "ez", "hx", "hy", "hz", "dx", "dy", "dz")
-PUMP = dict(nx=96, ny=96, nz=64, steps=1500, fluence_mj_cm2=60.0, npml=8, src_z=60e-9, n_planes=0)
+PUMP = dict(nx=128, ny=128, nz=64, steps=1500, fluence_mj_cm2=60.0, npml=8, src_z=60e-9, n_planes=0)
 RELAX, RELAX_DT, RELAX_SUB = 600, 5e-15, 10
 _REF = {}
 
@@ -134,7 +134,7 @@ def errors(psi, ade):
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("kind", ["f32", "f16"])
 def test_storage_form_vs_fp64_oracle(kind):
-    print(f"case storage form {kind} vs FP64 oracle 96x96x64", flush=True)
+    print(f"case storage form {kind} vs FP64 oracle 128x128x64", flush=True)
     e = errors(kind, kind)
     print({k: f"{v:.3e}" for k, v in e.items()}, flush=True)
     assert 0.02 < e["switched_ref"] < 0.2
