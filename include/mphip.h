@@ -385,6 +385,31 @@ int mp_probe_trace_plane(const mp_yee *g, int32_t axis, int64_t i_pre, int64_t i
 int mp_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const double *area,
                   double *out, int64_t slot);
 
+/* The device reductions of one probe shot over the pre and post DFT planes (buffers as
+ * mp_probe_dft_accumulate fills them for planes of n1 x n2 cells and nb bins; only the
+ * buffer pointers of pre and post are read), written to out (device, 4*nb + 10 doubles):
+ *   out[2q], out[2q+1]              Sx_pre[q]  = sum((Ea*conj(Hb) - Eb*conj(Ha)) * dA)
+ *   out[2nb+2q], out[2nb+2q+1]      Sx_post[q]
+ *   out[4nb], out[4nb+1]            net_pre = sum(energy_pre * dA), net_post
+ *   out[4nb+2 .. 4nb+9]             sum(dft[., center_bin] * mode_w * dA) for post Ea,
+ *                                   post Eb, pre Ea, pre Eb (re, im each; the caller
+ *                                   divides by the mode norm)
+ * with dA = area and mode_w device arrays of n1*n2 doubles.  Fixed partition and
+ * reduction order: the result is the same on every call.  Replaces _plane_poynting,
+ * _plane_net_flux and _jones_overlap on host downloads in probe_shot
+ * (src/sim/ProbeReadout.jl:322-351, :362-420). */
+int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2,
+                         int32_t nb, int32_t center_bin, const double *mode_w, const double *area,
+                         double *out);
+
+/* Weighted plane sum of one field component (0..5 = Ex, Ey, Ez, Hx, Hy, Hz) on the plane
+ * `index` normal to `axis`: out[0] = sum(F * weight) over the plane cells in (n1, n2)
+ * order, weight = 1 when null (device, n1*n2 doubles).  Replaces the host loops of
+ * _mode_overlap (src/sim/Monitors.jl:350-368) and _mean_plane_components
+ * (Monitors.jl:461-473) behind Polarimetry and mode-resolved Transmission/Reflection. */
+int mp_plane_overlap(const mp_yee *g, int32_t component, int32_t axis, int64_t index,
+                     const double *weight, double *out);
+
 /* nsteps of the fixed step sequence, all launches on the library stream, no host sync:
  * H -> source(t) -> E -> ADE x,y,z -> MO -> n,t += 1,dt -> pabs -> 4TM+LLB every
  * multiphysics_every steps over the window since the last consume -> DFT record.

@@ -300,6 +300,95 @@ static void plane_fields(const mp_yee *g, int axis, const void **Ea, const void 
     else { *Ea = g->ex; *Eb = g->ey; *Ha = g->hx; *Hb = g->hy; *n1 = g->nx; *n2 = g->ny; }
 }
 
+// ---- probe shot read-out (ProbeReadout.jl:322-351) ------------------------------------
+//
+// Grid (SHOT_BLOCKS, nb + 1).  Row q < nb reduces bin q's complex Poynting cross-power
+// sum((Ea*conj(Hb) - Eb*conj(Ha)) * dA) on both planes; row nb reduces the net flux
+// sum(energy * dA) on both planes and the four mode overlaps sum(dft[., q0] * w * dA) at
+// the centre bin (post Ea, post Eb, pre Ea, pre Eb).  Every row reads its own
+// contiguous (n1*n2) slab of each DFT array, so loads are coalesced 16-byte pairs.
+constexpr int SHOT_BLOCKS = 64;
+constexpr int SHOT_NV = 10;
+
+struct ShotPlane {
+    const double2 *ea, *eb, *ha, *hb;
+    const double *energy;
+};
+
+__device__ __forceinline__ void poynting_term(double2 a, double2 b, double2 ha, double2 hb, double dA, double &re,
+                                              double &im) {
+    // a*conj(hb) - b*conj(ha)
+    re += ((a.x * hb.x + a.y * hb.y) - (b.x * ha.x + b.y * ha.y)) * dA;
+    im += ((a.y * hb.x - a.x * hb.y) - (b.y * ha.x - b.x * ha.y)) * dA;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void shot_partial_kernel(ShotPlane pre, ShotPlane post,
+                                                                   const double *mode_w, const double *area,
+                                                                   int64_t np, int nb, int q0, double *part) {
+    const int q = blockIdx.y;
+    double v[SHOT_NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t c0 = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x;
+    const int64_t cs = (int64_t)SHOT_BLOCKS * RED_THREADS;
+    if (q < nb) {
+        const int64_t o = (int64_t)q * np;
+        for (int64_t c = c0; c < np; c += cs) {
+            const double dA = area[c];
+            poynting_term(pre.ea[o + c], pre.eb[o + c], pre.ha[o + c], pre.hb[o + c], dA, v[0], v[1]);
+            poynting_term(post.ea[o + c], post.eb[o + c], post.ha[o + c], post.hb[o + c], dA, v[2], v[3]);
+        }
+    } else {
+        const int64_t o = (int64_t)q0 * np;
+        for (int64_t c = c0; c < np; c += cs) {
+            const double dA = area[c];
+            v[0] += pre.energy[c] * dA;
+            v[1] += post.energy[c] * dA;
+            const double w = mode_w[c] * dA;
+            const double2 fa = post.ea[o + c], fb = post.eb[o + c], ka = pre.ea[o + c], kb = pre.eb[o + c];
+            v[2] += fa.x * w; v[3] += fa.y * w;
+            v[4] += fb.x * w; v[5] += fb.y * w;
+            v[6] += ka.x * w; v[7] += ka.y * w;
+            v[8] += kb.x * w; v[9] += kb.y * w;
+        }
+    }
+    block_sum_store<SHOT_NV>(v, part + SHOT_NV * ((int64_t)q * SHOT_BLOCKS + blockIdx.x));
+}
+
+// One thread per output value, partials summed in block order.  Output layout:
+// [0, 2nb) Sx_pre (re, im), [2nb, 4nb) Sx_post, then the SHOT_NV values of row nb.
+__global__ void shot_finalize_kernel(const double *part, int nb, double *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nout = 4 * nb + SHOT_NV;
+    if (t >= nout) return;
+    int q, v;
+    if (t < 4 * nb) {
+        const int plane = t / (2 * nb), r = t % (2 * nb);
+        q = r / 2;
+        v = 2 * plane + (r & 1);
+    } else {
+        q = nb;
+        v = t - 4 * nb;
+    }
+    double s = 0.0;
+    for (int b = 0; b < SHOT_BLOCKS; ++b) s += part[SHOT_NV * ((int64_t)q * SHOT_BLOCKS + b) + v];
+    out[t] = s;
+}
+
+// Weighted plane sum of one field component: sum(F[plane cell] * w[cell]), w = 1 when
+// weight is null (Monitors.jl:350-368 and :461-473).
+template <typename S>
+__global__ __launch_bounds__(RED_THREADS) void overlap_partial_kernel(const S *F, int axis, int64_t idx,
+                                                                      const double *weight, int64_t n1, int64_t n2,
+                                                                      int64_t nx, int64_t ny, double *part) {
+    double v[1] = {0};
+    const int64_t np = n1 * n2;
+    for (int64_t c = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; c < np; c += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const int64_t a = c % n1, b = c / n1;
+        const double f = (double)F[plane_lin(axis, idx, a, b, nx, ny)];
+        v[0] += weight ? f * weight[c] : f;
+    }
+    block_sum_store<1>(v, part + blockIdx.x);
+}
+
 }  // namespace mp
 
 using namespace mp;
@@ -499,6 +588,94 @@ MP_API int mp_flux_plane(const mp_yee *g, int32_t axis, int64_t index, const dou
     }
     MP_LAUNCH_CHECK("flux_partial_kernel");
     finalize_kernel<<<1, 64, 0, stream()>>>(part, 1, 1, 1, 1.0, out + slot);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+namespace mp {
+
+static double *g_shot_part = nullptr;  // SHOT_NV * SHOT_BLOCKS * (nb + 1) doubles
+static int64_t g_shot_cap = 0;
+
+static int shot_partials(int nb, double **out) {
+    std::lock_guard<std::mutex> lk(g_red_mu);
+    const int64_t need = (int64_t)SHOT_NV * SHOT_BLOCKS * (nb + 1);
+    if (need > g_shot_cap) {
+        if (g_shot_part) MP_HIP(hipFree(g_shot_part));
+        g_shot_part = nullptr;
+        g_shot_cap = 0;
+        MP_HIP(hipMalloc((void **)&g_shot_part, sizeof(double) * need));
+        g_shot_cap = need;
+    }
+    *out = g_shot_part;
+    return MP_OK;
+}
+
+static int shot_plane(const mp_plane *p, ShotPlane &s) {
+    if (!p->dft_ea || !p->dft_eb || !p->dft_ha || !p->dft_hb || !p->energy)
+        return fail_arg("mp_probe_shot_reduce: null plane buffer");
+    s.ea = (const double2 *)p->dft_ea;
+    s.eb = (const double2 *)p->dft_eb;
+    s.ha = (const double2 *)p->dft_ha;
+    s.hb = (const double2 *)p->dft_hb;
+    s.energy = p->energy;
+    return MP_OK;
+}
+
+}  // namespace mp
+
+MP_API int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2, int32_t nb,
+                                int32_t center_bin, const double *mode_w, const double *area, double *out) {
+    if (!pre || !post || !mode_w || !area || !out) return fail_arg("mp_probe_shot_reduce: null argument");
+    if (n1 < 1 || n2 < 1) return fail_arg("mp_probe_shot_reduce: plane sizes must be positive");
+    if (nb < 1 || nb > 65535) return fail_arg("mp_probe_shot_reduce: nb must be in [1, 65535]");
+    if (center_bin < 0 || center_bin >= nb) return fail_arg("mp_probe_shot_reduce: center_bin out of range");
+    const int64_t np = n1 * n2;
+    ShotPlane a, b;
+    int rc = shot_plane(pre, a);
+    if (rc) return rc;
+    rc = shot_plane(post, b);
+    if (rc) return rc;
+    double *part;
+    rc = shot_partials(nb, &part);
+    if (rc) return rc;
+    KernelTimer t("probe_shot_reduce");
+    shot_partial_kernel<<<dim3(SHOT_BLOCKS, nb + 1), RED_THREADS, 0, stream()>>>(a, b, mode_w, area, np, nb,
+                                                                                   center_bin, part);
+    MP_LAUNCH_CHECK("shot_partial_kernel");
+    const int nout = 4 * nb + SHOT_NV;
+    shot_finalize_kernel<<<(unsigned)cdiv(nout, 256), 256, 0, stream()>>>(part, nb, out);
+    MP_LAUNCH_CHECK("shot_finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_plane_overlap(const mp_yee *g, int32_t component, int32_t axis, int64_t index, const double *weight,
+                            double *out) {
+    if (!g || !out) return fail_arg("mp_plane_overlap: null argument");
+    if (component < 0 || component > 5) return fail_arg("mp_plane_overlap: component must be 0..5 (Ex..Hz)");
+    if (axis < 0 || axis > 2) return fail_arg("mp_plane_overlap: bad axis");
+    const int64_t nax = axis == 0 ? g->nx : (axis == 1 ? g->ny : g->nz);
+    if (index < 0 || index >= nax) return fail_arg("mp_plane_overlap: plane index out of range");
+    const void *comps[6] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz};
+    const void *F = comps[component];
+    if (!F) return fail_arg("mp_plane_overlap: null field");
+    const int64_t n1 = axis == 0 ? g->ny : (axis == 1 ? g->nz : g->nx);
+    const int64_t n2 = axis == 0 ? g->nz : (axis == 1 ? g->nx : g->ny);
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("plane_overlap");
+    if (g->field_dtype == MP_F32) {
+        overlap_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)F, axis, index, weight,
+                                                                                n1, n2, g->nx, g->ny, part);
+    } else if (g->field_dtype == MP_F64) {
+        overlap_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)F, axis, index,
+                                                                                 weight, n1, n2, g->nx, g->ny, part);
+    } else {
+        return fail_arg("mp_plane_overlap: unsupported field dtype");
+    }
+    MP_LAUNCH_CHECK("overlap_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, 1, 1, 1, 1.0, out);
     MP_LAUNCH_CHECK("finalize_kernel");
     return MP_OK;
 }

@@ -500,10 +500,9 @@ static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
     return MP_OK;
 }
 
-// Vector-form launch shape: 64 lanes x TYV rows per workgroup, z chunks of MP_YEE_KCHUNK
-// planes (default 1: on the 512 x 512 x 256 step, chunks of 1 / 2 / 4 / 8 planes give an
-// E update of 793 / 817 / 840 / 880 us).
-constexpr int TYV = 4;
+// Vector-form launch shape: z chunks of MP_YEE_KCHUNK planes per workgroup (default 1: on
+// the 512 x 512 x 256 step, chunks of 1 / 2 / 4 / 8 planes give an E update of
+// 793 / 817 / 840 / 880 us).
 static int64_t vec_kchunk() {
     static int64_t kc = [] {
         const char *e = getenv("MP_YEE_KCHUNK");

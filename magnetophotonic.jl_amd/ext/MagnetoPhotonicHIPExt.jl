@@ -363,6 +363,29 @@ function MagnetoPhotonic._ka_probe_dft_accumulate!(::HIPBackend, fields,
     return nothing
 end
 
+# probe_shot's plane reductions on the device (ProbeReadout.jl:322-351): one call, 4nb+10
+# doubles downloaded instead of the ten plane buffers.  Returns the values probe_shot
+# derives its T/R/A, spectra and Jones angles from.
+function probe_shot_reductions(m::MagnetoPhotonic.ProbeReadout)
+    nb = length(m.omega_bins)
+    pre = MpPlane(0, 0, 0, _p(m.dft_Ey_pre), _p(m.dft_Ez_pre), _p(m.dft_Hy_pre), _p(m.dft_Hz_pre), _p(m.energy_pre))
+    post = MpPlane(0, 0, 0, _p(m.dft_Ey_post), _p(m.dft_Ez_post), _p(m.dft_Hy_post), _p(m.dft_Hz_post),
+                   _p(m.energy_post))
+    out = HIPArray{Float64,1}((4nb + 10,))
+    _check(ccall((:mp_probe_shot_reduce, libmphip), Cint,
+                 (Ref{MpPlane}, Ref{MpPlane}, Int64, Int64, Int32, Int32, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                 Ref(pre), Ref(post), Int64(m.Ny), Int64(m.Nz), Int32(nb), Int32(m.center_bin - 1),
+                 _p(m.mode_w_dev), _p(m.area_yz_dev), _p(out)), "mp_probe_shot_reduce")
+    r = Array(out)
+    _free!(out)
+    Sx_pre = ComplexF64[complex(r[2q - 1], r[2q]) for q in 1:nb]
+    Sx_post = ComplexF64[complex(r[2nb + 2q - 1], r[2nb + 2q]) for q in 1:nb]
+    inv = 1.0 / m.mode_norm
+    j(i) = complex(r[4nb + 2 + 2i - 1], r[4nb + 2 + 2i]) * inv
+    return (; Sx_pre, Sx_post, net_pre=r[4nb + 1], net_post=r[4nb + 2],
+            jEy_far=j(1), jEz_far=j(2), jEy_kerr=j(3), jEz_kerr=j(4))
+end
+
 function __init__()
     ok = Libc.Libdl.dlopen(libmphip; throw_error=false) !== nothing &&
          ccall((:mp_init, libmphip), Cint, (Cint,), 0) == 0

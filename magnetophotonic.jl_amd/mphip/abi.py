@@ -110,6 +110,8 @@ SIGNATURES = {
     "probe_dft_accumulate": (C.c_int, [P(Yee), P(Plane), i32, vp, vp, i32, f64, f64]),
     "probe_trace_plane": (C.c_int, [P(Yee), i32, i64, i64, vp, vp, vp]),
     "flux_plane": (C.c_int, [P(Yee), i32, i64, vp, vp, i64]),
+    "probe_shot_reduce": (C.c_int, [P(Plane), P(Plane), i64, i64, i32, i32, vp, vp, vp]),
+    "plane_overlap": (C.c_int, [P(Yee), i32, i32, i64, vp, vp]),
     "film_reduce": (C.c_int, [P(Film), vp]),
     "last_error": (C.c_char_p, []),
 }

@@ -291,3 +291,101 @@ class OracleState:
 
     def set_film(self, key, values):
         self.fa[key][...] = np.asarray(values).astype(self.spec.compute_dtype)
+
+
+class OracleDFT:
+    """Host twin of mphip.monitors.DFTPlanes over the oracle (orc_probe_dft_accumulate)."""
+
+    def __init__(self, ref, planes, omega_bins):
+        from mphip import abi
+        from mphip.monitors import _plane_dims
+        self.ref, self.planes, self.nb = ref, [(int(a), int(i)) for a, i in planes], len(omega_bins)
+        self.omega = np.asarray(omega_bins, np.float64)
+        self.phase = np.zeros(2 * self.nb)
+        self.arrs = []
+        self.desc = (abi.Plane * max(1, len(self.planes)))()
+        for p, (axis, idx) in enumerate(self.planes):
+            n1, n2 = _plane_dims(ref.spec.shape, axis)
+            a = {k: np.zeros(2 * n1 * n2 * self.nb) for k in ("ea", "eb", "ha", "hb")}
+            a["energy"] = np.zeros(n1 * n2)
+            self.arrs.append((a, n1, n2))
+            d = self.desc[p]
+            d.axis, d.index = axis, idx
+            d.dft_ea, d.dft_eb, d.dft_ha, d.dft_hb = (a[k].ctypes.data for k in ("ea", "eb", "ha", "hb"))
+            d.energy = a["energy"].ctypes.data
+
+    def record(self, ref=None, t=None, dt=None):
+        ref = ref or self.ref
+        rc = ref.L.orc_probe_dft_accumulate(C.byref(ref.yee), self.desc, len(self.planes),
+                                            self.omega.ctypes.data_as(C.c_void_p),
+                                            self.phase.ctypes.data_as(C.c_void_p), self.nb,
+                                            ref.t if t is None else float(t), ref.dt if dt is None else float(dt))
+        assert rc == 0
+
+    def download(self, p):
+        a, n1, n2 = self.arrs[p]
+        out = {k: (a[k][0::2] + 1j * a[k][1::2]).reshape((n1, n2, self.nb), order="F")
+               for k in ("ea", "eb", "ha", "hb")}
+        out["energy"] = a["energy"].reshape((n1, n2), order="F")
+        return out
+
+    def raw(self, p):
+        """The flat accumulator arrays of plane p (interleaved re/im), as the device holds them."""
+        return self.arrs[p][0]
+
+
+class OracleProbe:
+    """Host twin of mphip.probe.ProbeReadout: DFT and traces recorded by the oracle after
+    every oracle step, plane reductions by orc_probe_shot_reduce; the scalar algebra is the
+    library's shot_from_reductions applied to these reductions."""
+
+    def __init__(self, ref, readout):
+        from mphip.monitors import plane_area
+        from mphip.sources import source_index
+        g = ref.spec.grid
+        self.ref, self.ro = ref, readout
+        self.i_pre = source_index(g.x, readout.x_pre)
+        self.i_post = source_index(g.x, readout.x_post)
+        self.area = plane_area(g, 0)
+        ny, nz = g.y.n, g.z.n
+        m = readout.mode
+        have = m is not None and np.shape(m) == (ny, nz)
+        self.mode_w = np.asarray(m, np.float64).reshape(-1, order="F") if have else np.ones(ny * nz)
+        norm = float(np.sum(np.abs(self.mode_w) * self.area))
+        self.mode_norm = norm if norm > 0.0 else 1.0
+        self.dft = OracleDFT(ref, [(0, self.i_pre), (0, self.i_post)], readout.omega_bins)
+        self.t_trace, self.traces = [], []
+
+    def record(self):
+        ref = self.ref
+        self.dft.record(ref)
+        if self.ro.trace_due(ref.n):
+            out = np.zeros(4)
+            rc = ref.L.orc_probe_trace_plane(C.byref(ref.yee), 0, self.i_pre, self.i_post, _ptr(self.mode_w),
+                                             _ptr(self.area), _ptr(out))
+            assert rc == 0
+            self.t_trace.append(ref.t)
+            self.traces.append(out / self.mode_norm)
+
+    def run(self, steps):
+        for _ in range(int(steps)):
+            self.ref.step()
+            self.record()
+
+    def reductions_raw(self):
+        nb = self.dft.nb
+        out = np.zeros(4 * nb + 10)
+        g = self.ref.spec.grid
+        rc = self.ref.L.orc_probe_shot_reduce(C.byref(self.dft.desc[0]), C.byref(self.dft.desc[1]), g.y.n, g.z.n,
+                                              nb, self.ro.center_bin, _ptr(self.mode_w), _ptr(self.area), _ptr(out))
+        assert rc == 0
+        return out
+
+
+def plane_overlap(ref, component, axis, index, weight=None):
+    """orc_plane_overlap on the oracle state's fields."""
+    out = np.zeros(1)
+    w = None if weight is None else np.ascontiguousarray(weight, np.float64)
+    rc = ref.L.orc_plane_overlap(C.byref(ref.yee), int(component), int(axis), int(index), _ptr(w), _ptr(out))
+    assert rc == 0
+    return float(out[0])

@@ -15,6 +15,8 @@
  *   src/fdtd/Kernels.jl:420-478  DFT phase/accumulate, trace  -> below
  *   src/fdtd/Source.jl:188-236   plane-source host loop        -> below
  *   src/sim/Monitors.jl:263-341  FluxMonitor plane integral    -> below
+ *   src/sim/ProbeReadout.jl:322-351  probe_shot plane reductions -> below
+ *   src/sim/Monitors.jl:350-368, 461-473  mode overlap / plane mean -> below
  *
  * Built with -O2 -ffp-contract=off -fno-fast-math -fopenmp (denormals kept).
  */
@@ -501,6 +503,76 @@ ORC_EXPORT int orc_flux_plane(const mp_yee *g, int32_t axis, int64_t index, cons
                     area[a + n1 * b];
         }
     out[slot] = flux;
+    return 0;
+}
+
+/* probe_shot's plane reductions (ProbeReadout.jl:322-351): per-bin complex cross-power
+ * _plane_poynting, _plane_net_flux and _jones_overlap (before the division by the mode
+ * norm), each summed over the plane in column-major cell order as the reference loops. */
+static void orc_poynting(const mp_plane *p, int64_t np, int q, const double *area, double *re, double *im) {
+    double sr = 0.0, si = 0.0;
+    for (int64_t c = 0; c < np; ++c) {
+        const int64_t z = 2 * (c + np * q);
+        const double ar = p->dft_ea[z], ai = p->dft_ea[z + 1], br = p->dft_eb[z], bi = p->dft_eb[z + 1];
+        const double har = p->dft_ha[z], hai = p->dft_ha[z + 1], hbr = p->dft_hb[z], hbi = p->dft_hb[z + 1];
+        /* (a * conj(hb) - b * conj(ha)) * dA */
+        sr += ((ar * hbr + ai * hbi) - (br * har + bi * hai)) * area[c];
+        si += ((ai * hbr - ar * hbi) - (bi * har - br * hai)) * area[c];
+    }
+    *re = sr;
+    *im = si;
+}
+
+static void orc_overlap_bin(const double *dft, int64_t np, int q, const double *mode_w, const double *area,
+                            double *out) {
+    double sr = 0.0, si = 0.0;
+    for (int64_t c = 0; c < np; ++c) {
+        const double w = mode_w[c] * area[c];
+        sr += dft[2 * (c + np * q)] * w;
+        si += dft[2 * (c + np * q) + 1] * w;
+    }
+    out[0] = sr;
+    out[1] = si;
+}
+
+ORC_EXPORT int orc_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2,
+                                     int32_t nb, int32_t center_bin, const double *mode_w, const double *area,
+                                     double *out) {
+    const int64_t np = n1 * n2;
+    for (int q = 0; q < nb; ++q) {
+        orc_poynting(pre, np, q, area, &out[2 * q], &out[2 * q + 1]);
+        orc_poynting(post, np, q, area, &out[2 * nb + 2 * q], &out[2 * nb + 2 * q + 1]);
+    }
+    double npre = 0.0, npost = 0.0;
+    for (int64_t c = 0; c < np; ++c) {
+        npre += pre->energy[c] * area[c];
+        npost += post->energy[c] * area[c];
+    }
+    out[4 * nb] = npre;
+    out[4 * nb + 1] = npost;
+    orc_overlap_bin(post->dft_ea, np, center_bin, mode_w, area, &out[4 * nb + 2]);
+    orc_overlap_bin(post->dft_eb, np, center_bin, mode_w, area, &out[4 * nb + 4]);
+    orc_overlap_bin(pre->dft_ea, np, center_bin, mode_w, area, &out[4 * nb + 6]);
+    orc_overlap_bin(pre->dft_eb, np, center_bin, mode_w, area, &out[4 * nb + 8]);
+    return 0;
+}
+
+/* _mode_overlap (Monitors.jl:350-368) / the sum of _mean_plane_components
+ * (Monitors.jl:461-473): sum(F * weight) over one plane, weight 1 when null. */
+ORC_EXPORT int orc_plane_overlap(const mp_yee *g, int32_t component, int32_t axis, int64_t index,
+                                 const double *weight, double *out) {
+    int64_t n1, n2;
+    const void *Ea, *Eb, *Ha, *Hb;
+    plane_geom(g, axis, &n1, &n2, &Ea, &Eb, &Ha, &Hb);
+    const void *comps[6] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz};
+    const void *F = comps[component];
+    double s = 0.0;
+    for (int64_t b = 0; b < n2; ++b)
+        for (int64_t a = 0; a < n1; ++a) {
+            const double f = load_f(F, g->field_dtype, plane_lin(g, axis, index, a, b));
+            s += weight ? f * weight[a + n1 * b] : f;
+        }
+    out[0] = s;
     return 0;
 }
 

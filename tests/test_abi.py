@@ -107,6 +107,21 @@ def test_argument_errors_before_device_work():
     assert lib.mp_multiphysics_step(C.byref(f), prm.ctypes.data_as(C.c_void_p), C.byref(lut), 1e-17, 0, 2, 1) == -1
     assert b"subcycles must be positive" in lib.mp_last_error()
     assert lib.mp_probe_trace_plane(C.byref(y), 3, 0, 0, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8)) == -1
+    pl = abi.Plane()
+    pl.dft_ea = pl.dft_eb = pl.dft_ha = pl.dft_hb = pl.energy = 8
+    v8 = C.c_void_p(8)
+    assert lib.mp_probe_shot_reduce(C.byref(pl), C.byref(pl), 4, 4, 0, 0, v8, v8, v8) == -1
+    assert b"nb must be" in lib.mp_last_error()
+    assert lib.mp_probe_shot_reduce(C.byref(pl), C.byref(pl), 4, 4, 5, 5, v8, v8, v8) == -1
+    assert b"center_bin" in lib.mp_last_error()
+    assert lib.mp_probe_shot_reduce(C.byref(pl), C.byref(pl), 0, 4, 5, 2, v8, v8, v8) == -1
+    pl.energy = None
+    assert lib.mp_probe_shot_reduce(C.byref(pl), C.byref(pl), 4, 4, 5, 2, v8, v8, v8) == -1
+    assert b"null plane buffer" in lib.mp_last_error()
+    assert lib.mp_plane_overlap(C.byref(y), 6, 0, 0, None, v8) == -1
+    assert b"component" in lib.mp_last_error()
+    assert lib.mp_plane_overlap(C.byref(y), 1, 2, 8, None, v8) == -1
+    assert b"out of range" in lib.mp_last_error()
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU is present")

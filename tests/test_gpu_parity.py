@@ -107,42 +107,7 @@ def test_coupled_pump_mo_dft_small():
     gpu.free()
 
 
-class OracleDFT:
-    """Host twin of DFTPlanes over the oracle."""
-
-    def __init__(self, ref, planes, omega_bins):
-        import ctypes as C
-        from mphip import abi
-        from mphip.monitors import _plane_dims
-        self.ref, self.planes, self.nb = ref, planes, len(omega_bins)
-        self.omega = np.asarray(omega_bins, np.float64)
-        self.phase = np.zeros(2 * self.nb)
-        self.arrs = []
-        self.desc = (abi.Plane * len(planes))()
-        for p, (axis, idx) in enumerate(planes):
-            n1, n2 = _plane_dims(ref.spec.shape, axis)
-            a = {k: np.zeros(2 * n1 * n2 * self.nb) for k in ("ea", "eb", "ha", "hb")}
-            a["energy"] = np.zeros(n1 * n2)
-            self.arrs.append((a, n1, n2))
-            d = self.desc[p]
-            d.axis, d.index = axis, idx
-            d.dft_ea, d.dft_eb, d.dft_ha, d.dft_hb = (a[k].ctypes.data for k in ("ea", "eb", "ha", "hb"))
-            d.energy = a["energy"].ctypes.data
-        self.C = C
-
-    def record(self, ref):
-        C = self.C
-        rc = ref.L.orc_probe_dft_accumulate(C.byref(ref.yee), self.desc, len(self.planes),
-                                            self.omega.ctypes.data_as(C.c_void_p),
-                                            self.phase.ctypes.data_as(C.c_void_p), self.nb, ref.t, ref.dt)
-        assert rc == 0
-
-    def download(self, p):
-        a, n1, n2 = self.arrs[p]
-        out = {k: (a[k][0::2] + 1j * a[k][1::2]).reshape((n1, n2, self.nb), order="F")
-               for k in ("ea", "eb", "ha", "hb")}
-        out["energy"] = a["energy"].reshape((n1, n2), order="F")
-        return out
+from driver import OracleDFT  # noqa: E402
 
 
 def test_soft_and_mode_sources_bitwise():
