@@ -469,6 +469,23 @@ int mp_surrogate_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
  * doubles. */
 int mp_film_reduce(const mp_film *f, double *out);
 
+/* The sums behind one FilmAverage / SwitchedFraction record (src/sim/Monitors.jl:
+ * 518-570, 487-497; reduce_mean / reduce_norm_mean / reduce_count_negative,
+ * src/core/Backend.jl:48-57), over the f->n film cells in FP64: out[0..5] = sum of
+ * m_TM_x, m_TM_y, m_TM_z, m_RE_x, m_RE_y, m_RE_z; out[6], out[7] = sum of |m_TM|, |m_RE|;
+ * out[8..11] = sum of Te, Tl, Ts_TM, Ts_RE; out[12] = count of m_TM_x < 0.  The caller
+ * divides by n.  out: 13 device doubles (zeros when n == 0).  Fixed reduction order. */
+int mp_film_average(const mp_film *f, double *out);
+
+/* Final-state switching diagnostics (switching_metrics, src/sim/Monitors.jl:682-727):
+ * a cell is switched when m_TM_x < 0 and m_RE_x > 0; the core is U_abs >=
+ * core_fraction * max(U_abs).  out (13 device doubles): [0] switched count, [1] count
+ * m_TM_x < 0, [2] count m_RE_x > 0, [3], [4] sums of m_TM_x, m_RE_x over switched cells,
+ * [5], [6] the same over unswitched cells, [7] max U_abs, [8] its first 0-based index,
+ * [9] core count, [10] switched core count, [11], [12] sums of m_TM_x, m_RE_x over the
+ * core.  The caller forms the fractions and means.  n must be positive. */
+int mp_switching_metrics(const mp_film *f, double core_fraction, double *out);
+
 /* Non-finite scan of a device array (NaNGuard, Monitors.jl:580-606): *flag (device
  * int32) is set to 1 if any element is NaN or Inf; the caller clears it. */
 int mp_any_nonfinite(const void *x, int32_t dtype, int64_t n, int32_t *flag);

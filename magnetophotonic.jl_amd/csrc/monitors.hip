@@ -184,7 +184,7 @@ static void launch_dft_batch(dim3 grid, const PlaneSet &ps, const void *snap, in
 
 constexpr int RED_BLOCKS = 256;
 constexpr int RED_THREADS = 256;
-constexpr int RED_MAXV = 10;
+constexpr int RED_MAXV = 16;
 
 static double *g_partials = nullptr;  // RED_BLOCKS * RED_MAXV doubles
 static std::mutex g_red_mu;
@@ -387,6 +387,110 @@ __global__ __launch_bounds__(RED_THREADS) void overlap_partial_kernel(const S *F
         v[0] += weight ? f * weight[c] : f;
     }
     block_sum_store<1>(v, part + blockIdx.x);
+}
+
+
+// ---- film averages and switching diagnostics ------------------------------------------------
+//
+// FilmAverage (Monitors.jl:518-570): per-block sums of the six m components, |m_TM|,
+// |m_RE|, the four temperatures and the count of m_TM_x < 0 (13 values).
+template <typename L>
+__global__ __launch_bounds__(RED_THREADS) void film_average_partial_kernel(
+    const L *mTx, const L *mTy, const L *mTz, const L *mRx, const L *mRy, const L *mRz, const L *Te, const L *Tl,
+    const L *TsT, const L *TsR, int64_t N, double *part) {
+    double v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; n < N; n += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const double tx = (double)mTx[n], ty = (double)mTy[n], tz = (double)mTz[n];
+        const double rx = (double)mRx[n], ry = (double)mRy[n], rz = (double)mRz[n];
+        v[0] += tx; v[1] += ty; v[2] += tz;
+        v[3] += rx; v[4] += ry; v[5] += rz;
+        v[6] += sqrt(tx * tx + ty * ty + tz * tz);
+        v[7] += sqrt(rx * rx + ry * ry + rz * rz);
+        v[8] += (double)Te[n]; v[9] += (double)Tl[n]; v[10] += (double)TsT[n]; v[11] += (double)TsR[n];
+        v[12] += mTx[n] < (L)0 ? 1.0 : 0.0;
+    }
+    block_sum_store<13>(v, part + RED_MAXV * blockIdx.x);
+}
+
+// switching_metrics pass 1 (Monitors.jl:682-727): switched = m_TM_x < 0 and m_RE_x > 0.
+// Sums: switched count, TM-reversed count, RE-reversed count, m_TM_x and m_RE_x over
+// switched and over unswitched cells (7); then max U_abs and its first index.
+template <typename L>
+__global__ __launch_bounds__(RED_THREADS) void switch_partial_kernel(const L *mTx, const L *mRx, const double *U,
+                                                                      int64_t N, double *part) {
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    double umax = -INFINITY;
+    int64_t imax = N;
+    for (int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; n < N; n += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const double tm = (double)mTx[n], re = (double)mRx[n];
+        const bool sw = tm < 0.0 && re > 0.0;
+        v[0] += sw ? 1.0 : 0.0;
+        v[1] += tm < 0.0 ? 1.0 : 0.0;
+        v[2] += re > 0.0 ? 1.0 : 0.0;
+        if (sw) { v[3] += tm; v[4] += re; } else { v[5] += tm; v[6] += re; }
+        const double u = U[n];
+        if (u > umax || imax == N) { umax = u; imax = n; }   // n rises per thread: first max kept
+    }
+    block_sum_store<7>(v, part + RED_MAXV * blockIdx.x);
+    __shared__ double mv[RED_THREADS];
+    __shared__ int64_t mi[RED_THREADS];
+    mv[threadIdx.x] = umax;
+    mi[threadIdx.x] = imax;
+    __syncthreads();
+    for (int st = RED_THREADS / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) {
+            const double ov = mv[threadIdx.x + st];
+            const int64_t oi = mi[threadIdx.x + st];
+            if (oi < N && (mi[threadIdx.x] == N || ov > mv[threadIdx.x] || (ov == mv[threadIdx.x] && oi < mi[threadIdx.x]))) {
+                mv[threadIdx.x] = ov;
+                mi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[RED_MAXV * blockIdx.x + 7] = mv[0];
+        part[RED_MAXV * blockIdx.x + 8] = (double)mi[0];
+    }
+}
+
+// Ordered sums of the 7 pass-1 values, and the first maximum over blocks: out[0..8].
+__global__ void switch_finalize_kernel(const double *part, int64_t N, double *out) {
+    const int q = threadIdx.x;
+    if (q < 7) {
+        double s = 0.0;
+        for (int b = 0; b < RED_BLOCKS; ++b) s += part[RED_MAXV * b + q];
+        out[q] = s;
+    } else if (q == 7) {
+        double m = -INFINITY;
+        double mi = (double)N;
+        for (int b = 0; b < RED_BLOCKS; ++b) {
+            const double v = part[RED_MAXV * b + 7], i = part[RED_MAXV * b + 8];
+            if (i < (double)N && (mi == (double)N || v > m || (v == m && i < mi))) { m = v; mi = i; }
+        }
+        out[7] = m;
+        out[8] = mi;
+    }
+}
+
+// switching_metrics pass 2: the core is U_abs >= core_fraction * max(U_abs); counts of core
+// and core-and-switched cells, m_TM_x and m_RE_x summed over the core.
+template <typename L>
+__global__ __launch_bounds__(RED_THREADS) void core_partial_kernel(const L *mTx, const L *mRx, const double *U,
+                                                                    int64_t N, const double *stats, double core_fraction,
+                                                                    double *part) {
+    const double thr = core_fraction * stats[7];
+    double v[4] = {0, 0, 0, 0};
+    for (int64_t n = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; n < N; n += (int64_t)RED_BLOCKS * RED_THREADS) {
+        if (U[n] >= thr) {
+            const double tm = (double)mTx[n], re = (double)mRx[n];
+            v[0] += 1.0;
+            v[1] += (tm < 0.0 && re > 0.0) ? 1.0 : 0.0;
+            v[2] += tm;
+            v[3] += re;
+        }
+    }
+    block_sum_store<4>(v, part + RED_MAXV * blockIdx.x);
 }
 
 }  // namespace mp
@@ -700,6 +804,65 @@ MP_API int mp_film_reduce(const mp_film *f, double *out) {
     }
     MP_LAUNCH_CHECK("film_partial_kernel");
     finalize_kernel<<<1, 64, 0, stream()>>>(part, RED_MAXV, 9, 10, 1.0 / (double)f->n, out);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_film_average(const mp_film *f, double *out) {
+    if (!f || !out) return fail_arg("mp_film_average: null argument");
+    if (f->n <= 0) return mp_fill(out, MP_F64, 13, 0.0);
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("film_average");
+    if (f->dtype == MP_F32) {
+        film_average_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)f->m_TM_x,
+            (const float *)f->m_TM_y, (const float *)f->m_TM_z, (const float *)f->m_RE_x, (const float *)f->m_RE_y,
+            (const float *)f->m_RE_z, (const float *)f->Te, (const float *)f->Tl, (const float *)f->Ts_TM,
+            (const float *)f->Ts_RE, f->n, part);
+    } else if (f->dtype == MP_F64) {
+        film_average_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)f->m_TM_x,
+            (const double *)f->m_TM_y, (const double *)f->m_TM_z, (const double *)f->m_RE_x, (const double *)f->m_RE_y,
+            (const double *)f->m_RE_z, (const double *)f->Te, (const double *)f->Tl, (const double *)f->Ts_TM,
+            (const double *)f->Ts_RE, f->n, part);
+    } else {
+        return fail_arg("mp_film_average: unsupported film dtype");
+    }
+    MP_LAUNCH_CHECK("film_average_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, RED_MAXV, 13, 13, 1.0, out);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_switching_metrics(const mp_film *f, double core_fraction, double *out) {
+    if (!f || !out) return fail_arg("mp_switching_metrics: null argument");
+    if (f->n <= 0) return fail_arg("mp_switching_metrics: empty film");
+    if (!f->U_abs) return fail_arg("mp_switching_metrics: null U_abs");
+    double *part;
+    int rc = partials(&part);
+    if (rc) return rc;
+    KernelTimer t("switching_metrics");
+    if (f->dtype == MP_F32) {
+        switch_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)f->m_TM_x,
+            (const float *)f->m_RE_x, f->U_abs, f->n, part);
+    } else if (f->dtype == MP_F64) {
+        switch_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)f->m_TM_x,
+            (const double *)f->m_RE_x, f->U_abs, f->n, part);
+    } else {
+        return fail_arg("mp_switching_metrics: unsupported film dtype");
+    }
+    MP_LAUNCH_CHECK("switch_partial_kernel");
+    switch_finalize_kernel<<<1, 64, 0, stream()>>>(part, f->n, out);
+    MP_LAUNCH_CHECK("switch_finalize_kernel");
+    if (f->dtype == MP_F32) {
+        core_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)f->m_TM_x,
+            (const float *)f->m_RE_x, f->U_abs, f->n, out, core_fraction, part);
+    } else {
+        core_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)f->m_TM_x,
+            (const double *)f->m_RE_x, f->U_abs, f->n, out, core_fraction, part);
+    }
+    MP_LAUNCH_CHECK("core_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, RED_MAXV, 4, 4, 1.0, out + 9);
     MP_LAUNCH_CHECK("finalize_kernel");
     return MP_OK;
 }

@@ -113,6 +113,8 @@ SIGNATURES = {
     "probe_shot_reduce": (C.c_int, [P(Plane), P(Plane), i64, i64, i32, i32, vp, vp, vp]),
     "plane_overlap": (C.c_int, [P(Yee), i32, i32, i64, vp, vp]),
     "film_reduce": (C.c_int, [P(Film), vp]),
+    "film_average": (C.c_int, [P(Film), vp]),
+    "switching_metrics": (C.c_int, [P(Film), f64, vp]),
     "last_error": (C.c_char_p, []),
 }
 

@@ -301,6 +301,9 @@ class ProbeReadout:
             raise ValueError("a ProbeReadout records one state")
 
     # -- recording ----------------------------------------------------------------------
+    def due(self, n: int) -> bool:
+        return True
+
     def trace_due(self, n: int) -> bool:
         """n == 1, every trace_stride-th step, and the final step (ProbeReadout.jl:111-112)."""
         return n == 1 or n % self.trace_stride == 0 or (self.final_step > 0 and n == self.final_step)
@@ -396,6 +399,9 @@ class _PlaneSeries:
     def __init__(self, width: int):
         self.t = []
         self.slots = _Slots(width)
+
+    def due(self, n: int) -> bool:
+        return True
 
     def free(self):
         self.slots.free()

@@ -428,11 +428,11 @@ class FDTDState:
         return self
 
     def relax_step(self, dt: float, subcycles: Optional[int] = None):
-        """relax_step! (Solver.jl:548-555): 4TM+LLB with no heating."""
+        """relax_step! (Solver.jl:548-555): 4TM+LLB with no heating.  Like the reference it
+        leaves n and t alone; the relax phase driver (phases.run_relax) advances them."""
         if self.film is None:
             return self
         self._film_update(float(dt), int(subcycles or self.spec.subcycles), 0)
-        self.t += float(dt)
         return self
 
     # ---- host access -------------------------------------------------------------------------

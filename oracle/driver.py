@@ -268,7 +268,6 @@ class OracleState:
         self._check(self.L.orc_multiphysics_step(C.byref(self.film), _ptr(self.gd_arr), C.byref(self.lut),
                                                  float(dt), int(subcycles or self.spec.subcycles),
                                                  self.spec.brillouin_iters, 0), "relax")
-        self.t += float(dt)
         return self
 
     def run(self, steps):
@@ -290,7 +289,7 @@ class OracleState:
         return {k: self.fa[k].copy() for k in self.FILM_NAMES + ("U_abs", "window_sum", "peak")}
 
     def set_film(self, key, values):
-        self.fa[key][...] = np.asarray(values).astype(self.spec.compute_dtype)
+        self.fa[key][...] = np.asarray(values).astype(self.fa[key].dtype)
 
 
 class OracleDFT:
@@ -389,3 +388,17 @@ def plane_overlap(ref, component, axis, index, weight=None):
     rc = ref.L.orc_plane_overlap(C.byref(ref.yee), int(component), int(axis), int(index), _ptr(w), _ptr(out))
     assert rc == 0
     return float(out[0])
+
+
+def film_average(ref):
+    """orc_film_average on the oracle state's film: the 13 sums."""
+    out = np.zeros(13)
+    assert ref.L.orc_film_average(C.byref(ref.film), _ptr(out)) == 0
+    return out
+
+
+def switching_sums(ref, core_fraction=0.5):
+    """orc_switching_metrics on the oracle state's film: the 13 counts and sums."""
+    out = np.zeros(13)
+    assert ref.L.orc_switching_metrics(C.byref(ref.film), float(core_fraction), _ptr(out)) == 0
+    return out

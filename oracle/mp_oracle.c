@@ -593,6 +593,58 @@ ORC_EXPORT int orc_film_reduce(const mp_film *f, double *out) {
     return 0;
 }
 
+/* The sums of one FilmAverage / SwitchedFraction record (Monitors.jl:487-497, 518-570;
+ * reduce_mean, reduce_norm_mean, reduce_count_negative, Backend.jl:48-57). */
+ORC_EXPORT int orc_film_average(const mp_film *f, double *out) {
+    double s[13] = {0};
+    for (int64_t n = 0; n < f->n; ++n) {
+        const double tx = load_f(f->m_TM_x, f->dtype, n), ty = load_f(f->m_TM_y, f->dtype, n),
+                     tz = load_f(f->m_TM_z, f->dtype, n);
+        const double rx = load_f(f->m_RE_x, f->dtype, n), ry = load_f(f->m_RE_y, f->dtype, n),
+                     rz = load_f(f->m_RE_z, f->dtype, n);
+        s[0] += tx; s[1] += ty; s[2] += tz;
+        s[3] += rx; s[4] += ry; s[5] += rz;
+        s[6] += sqrt(tx * tx + ty * ty + tz * tz);
+        s[7] += sqrt(rx * rx + ry * ry + rz * rz);
+        s[8] += load_f(f->Te, f->dtype, n);
+        s[9] += load_f(f->Tl, f->dtype, n);
+        s[10] += load_f(f->Ts_TM, f->dtype, n);
+        s[11] += load_f(f->Ts_RE, f->dtype, n);
+        if (tx < 0.0) s[12] += 1.0;
+    }
+    memcpy(out, s, sizeof(s));
+    return 0;
+}
+
+/* switching_metrics (Monitors.jl:682-727) as the counts and sums of mp_switching_metrics */
+ORC_EXPORT int orc_switching_metrics(const mp_film *f, double core_fraction, double *out) {
+    if (f->n <= 0) return orc_fail("orc_switching_metrics: empty film");
+    double s[13] = {0};
+    double umax = -INFINITY;
+    int64_t imax = -1;
+    for (int64_t n = 0; n < f->n; ++n) {
+        const double tm = load_f(f->m_TM_x, f->dtype, n), re = load_f(f->m_RE_x, f->dtype, n);
+        const int sw = tm < 0.0 && re > 0.0;
+        s[0] += sw; s[1] += tm < 0.0; s[2] += re > 0.0;
+        if (sw) { s[3] += tm; s[4] += re; } else { s[5] += tm; s[6] += re; }
+        if (imax < 0 || f->U_abs[n] > umax) { umax = f->U_abs[n]; imax = n; }
+    }
+    s[7] = umax;
+    s[8] = (double)imax;
+    const double thr = core_fraction * umax;
+    for (int64_t n = 0; n < f->n; ++n) {
+        if (f->U_abs[n] >= thr) {
+            const double tm = load_f(f->m_TM_x, f->dtype, n), re = load_f(f->m_RE_x, f->dtype, n);
+            s[9] += 1.0;
+            s[10] += (tm < 0.0 && re > 0.0);
+            s[11] += tm;
+            s[12] += re;
+        }
+    }
+    memcpy(out, s, sizeof(s));
+    return 0;
+}
+
 /* Training rows of the surrogate (restates mp_surrogate_dataset). */
 ORC_EXPORT int orc_surrogate_dataset(const mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
                                      double dt, int32_t subcycles, int32_t iters, int32_t has_pabs,
