@@ -134,6 +134,36 @@ def test_soft_and_mode_sources_bitwise():
         gpu.free()
 
 
+def test_solved_mode_source_bitwise():
+    """A mode source whose profile comes from the scalar mode solver on the waveguide's
+    cross-section (Experiment.jl:149-165) injects through the same kernel, bit for bit."""
+    import math
+    import driver
+    from mphip import geometry, grid, sources
+    from mphip.modesolver import mode_source
+    from mphip.spec import build_state_spec
+    from mphip.state import FDTDState
+    d = 20e-9
+    g = grid.Grid3D(grid.uniform_axis_n(48, d), grid.uniform_axis_n(40, d), grid.uniform_axis_n(36, d))
+    sc = geometry.Scene()
+    sc.add(geometry.Box(0.0, 48 * d, 14 * d, 26 * d, 13 * d, 23 * d), geometry.Material("core", epsr=4.0))
+    geo = geometry.rasterize(sc, g, background=1.44 ** 2)
+    pulse = sources.GaussianPulse(amplitude=1e3, t0=4e-15, tau=1.5e-15, omega=2 * math.pi * grid.C0 / 800e-9)
+    src = mode_source(g, pulse, 8, 800e-9, epsr_volume=geo.epsr, component="Ez")
+    assert 1.44 < src.neff < 2.0
+    spec = build_state_spec(g, geo, grid.cfl_dt_cells(g, courant=0.5), source=src, n_pml=4)
+    print(f"case solved mode source neff={src.neff:.5f}", flush=True)
+    gpu = FDTDState(spec)
+    ref = driver.OracleState(spec)
+    gpu.run(80)
+    ref.run(80)
+    for name in FIELDS:
+        a, b = gpu.field(name), ref.field(name)
+        assert np.array_equal(a, b), (name, np.abs(a - b).max())
+    assert np.abs(ref.field("ez")).max() > 0
+    gpu.free()
+
+
 def test_relax_and_reductions():
     import driver
     from mphip import configs
