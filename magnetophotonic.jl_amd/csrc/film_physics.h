@@ -8,14 +8,37 @@
 
 namespace mp {
 
-__device__ __forceinline__ double brillouin(double S, double x) {
+// Uniform constants of the film update.  film_consts() derives them once on the host from
+// mp_ferri_params with the same IEEE double operations the per-cell code would apply to
+// them (no contraction on either side), so every value is identical to computing it in the
+// kernel.  The kernels read them from device memory through scalar loads, so no uniform
+// double occupies a vector register pair and no kernel carries the 97-field struct as
+// arguments.
+struct FilmConsts {
+    // update_Te_energy / update_4tm (Thermal.jl:24-40)
+    double gamma_e, half_gamma_e, Gel, Ges_TM, Ges_RE, inv_Cl, T0, inv_Cs_TM, inv_Cs_RE;
+    // llb_step (Magnetization.jl:105-263)
+    double T_Curie, inv_T_Curie, alpha0_TM, alpha0_RE, two_alpha0_TM, two_alpha0_RE;
+    double J_TMRE, kb, inv_mu_TM, inv_mu_RE, two_K_over_Ms_TM, two_K_over_Ms_RE;
+    double twelve_J_TMTM, twelve_J_RERE, gamma_gyro, Hsw0, Tsw, inv_dTsw, Ms_TM, Ms_RE;
+    // hot_weight centres: 0.92, 0.95, 0.90 and 0.88 x T_Curie
+    double Tw_scatter_TM, Tw_scatter_RE, Tw_cool_TM, Tw_cool_RE;
+    // brillouin(S, x) of each sublattice (0 = TM, 1 = RE): a = (2S+1)/(2S), b = 1/(2S) and
+    // the small-x series coefficients (a^2 - b^2)/3 and (a^4 - b^4)/45
+    double bri_a[2], bri_b[2], bri_c1[2], bri_c3[2];
+};
+
+// Device copy of the constants for `p` (cached per device; re-derived and uploaded only
+// when the parameters change, after the device has drained).  NULL on error.
+const FilmConsts *film_consts(const mp_ferri_params *p);
+
+// brillouin(S, x) (Magnetization.jl:7-22) with a, b and the series coefficients of the
+// sublattice taken from the constants (k = 0: TM, 1: RE).
+__device__ __forceinline__ double brillouin(const FilmConsts &c, int k, double x) {
     if (!__builtin_isfinite(x)) return 0.0;
-    const double a = (2.0 * S + 1.0) / (2.0 * S);
-    const double b = 1.0 / (2.0 * S);
-    if (__builtin_fabs(x) < 1e-4) {
-        const double a2 = a * a, b2 = b * b;
-        return ((a * a - b * b) / 3.0) * x - ((a2 * a2 - b2 * b2) / 45.0) * (x * x * x);
-    }
+    const double a = c.bri_a[k];
+    const double b = c.bri_b[k];
+    if (__builtin_fabs(x) < 1e-4) return c.bri_c1[k] * x - c.bri_c3[k] * (x * x * x);
     const double ax = a * x;
     const double bx = b * x;
     const double coth_ax = __builtin_fabs(ax) > 500.0 ? __builtin_copysign(1.0, ax) : (exp(2.0 * ax) + 1.0) / (exp(2.0 * ax) - 1.0);
@@ -54,7 +77,7 @@ __device__ __forceinline__ void damp_radial(double mx, double my, double mz, dou
 // llb_step (Magnetization.jl:105-263); m[0..5] in/out; returns dm_TM_x, dm_RE_x, Hani_TM_x, Hani_RE_x.
 template <typename L>
 __device__ __forceinline__ void llb_step(double m[6], double te_current, double ts_TM_old,
-                                         double ts_RE_old, const mp_ferri_params &gd,
+                                         double ts_RE_old, const FilmConsts &gd,
                                          const L *tm_lut, const L *re_lut, double lut_T_min,
                                          double lut_inv_dT, int32_t lut_N, double dt,
                                          int brillouin_iters, double out[4]) {
@@ -68,9 +91,9 @@ __device__ __forceinline__ void llb_step(double m[6], double te_current, double 
                                              : gd.alpha0_TM / (gd.T_Curie * jl_max(ratio_TM - 1.0, 0.01));
     const double chi_par_RE = ratio_RE < 1.0 ? gd.alpha0_RE / (gd.T_Curie * jl_max(1.0 - ratio_RE, 0.01))
                                              : gd.alpha0_RE / (gd.T_Curie * jl_max(ratio_RE - 1.0, 0.01));
-    const double alpha_par_TM = ratio_TM < 1.0 ? gd.alpha0_TM * 2.0 * jl_max(ratio_TM, 0.01) / 3.0 : gd.alpha0_TM;
+    const double alpha_par_TM = ratio_TM < 1.0 ? gd.two_alpha0_TM * jl_max(ratio_TM, 0.01) / 3.0 : gd.alpha0_TM;
     const double alpha_perp_TM = ratio_TM < 1.0 ? gd.alpha0_TM * (1.0 - jl_max(ratio_TM, 0.01) / 3.0) : gd.alpha0_TM;
-    const double alpha_par_RE = ratio_RE < 1.0 ? gd.alpha0_RE * 2.0 * jl_max(ratio_RE, 0.01) / 3.0 : gd.alpha0_RE;
+    const double alpha_par_RE = ratio_RE < 1.0 ? gd.two_alpha0_RE * jl_max(ratio_RE, 0.01) / 3.0 : gd.alpha0_RE;
     const double alpha_perp_RE = ratio_RE < 1.0 ? gd.alpha0_RE * (1.0 - jl_max(ratio_RE, 0.01) / 3.0) : gd.alpha0_RE;
     const double m2_TM = mx_TM * mx_TM + my_TM * my_TM + mz_TM * mz_TM;
     const double m2_RE = mx_RE * mx_RE + my_RE * my_RE + mz_RE * mz_RE;
@@ -80,7 +103,7 @@ __device__ __forceinline__ void llb_step(double m[6], double te_current, double 
     const double J_eff_TMRE = gd.J_TMRE * exchange_softening;
     double hot_window = 0.0;
     if (te_current > gd.T_Curie) {
-        const double u = (te_current - gd.Tsw) * (1.0 / gd.dTsw);
+        const double u = (te_current - gd.Tsw) * gd.inv_dTsw;
         hot_window = exp(-0.5 * (u * u));
     }
     const double Hsw = gd.Hsw0 * hot_window;
@@ -100,8 +123,8 @@ __device__ __forceinline__ void llb_step(double m[6], double te_current, double 
         for (int it = 0; it < brillouin_iters; ++it) {
             const double x_TM = (gd.twelve_J_TMTM * m_eq_TM + twelve_J_eff * m_eq_RE) * inv_kb_ts_TM;
             const double x_RE = (gd.twelve_J_RERE * m_eq_RE + twelve_J_eff * m_eq_TM) * inv_kb_ts_RE;
-            m_eq_TM = jl_clamp(brillouin(gd.S_TM, x_TM), -1.0, 1.0);
-            m_eq_RE = jl_clamp(brillouin(gd.S_RE, x_RE), -1.0, 1.0);
+            m_eq_TM = jl_clamp(brillouin(gd, 0, x_TM), -1.0, 1.0);
+            m_eq_RE = jl_clamp(brillouin(gd, 1, x_RE), -1.0, 1.0);
         }
         const double m_eq_floor = 0.01;
         Hlong_fac_TM = (__builtin_fabs(m_eq_TM) - mag_TM) / (2.0 * chi_par_TM * jl_max(__builtin_fabs(m_eq_TM), m_eq_floor));
@@ -135,10 +158,10 @@ __device__ __forceinline__ void llb_step(double m[6], double te_current, double 
     const double dm_RE_x_trans = -g * cx_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * mx_RE - g * alpha_perp_RE * inv_m2_RE * ccx_RE;
     const double dm_RE_y_trans = -g * cy_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * my_RE - g * alpha_perp_RE * inv_m2_RE * ccy_RE;
     const double dm_RE_z_trans = -g * cz_RE + g * alpha_par_RE * inv_m2_RE * mdotH_RE * mz_RE - g * alpha_perp_RE * inv_m2_RE * ccz_RE;
-    const double W_TM_scatter = hot_weight(te_current, 0.92 * gd.T_Curie, 1.0 / 40.0);
-    const double W_RE_scatter = hot_weight(ts_RE_old, 0.95 * gd.T_Curie, 1.0 / 75.0);
-    const double cool_TM = 1.0 - hot_weight(ts_TM_old, 0.90 * gd.T_Curie, 1.0 / 55.0);
-    const double cool_RE = 1.0 - hot_weight(ts_RE_old, 0.88 * gd.T_Curie, 1.0 / 75.0);
+    const double W_TM_scatter = hot_weight(te_current, gd.Tw_scatter_TM, 1.0 / 40.0);
+    const double W_RE_scatter = hot_weight(ts_RE_old, gd.Tw_scatter_RE, 1.0 / 75.0);
+    const double cool_TM = 1.0 - hot_weight(ts_TM_old, gd.Tw_cool_TM, 1.0 / 55.0);
+    const double cool_RE = 1.0 - hot_weight(ts_RE_old, gd.Tw_cool_RE, 1.0 / 75.0);
     const double tm_switched_gate = hot_weight(-mx_TM, 0.006, 1.0 / 0.010);
     const double re_res_floor = 0.28 * W_TM_scatter * (1.0 - 0.50 * W_RE_scatter);
     const double tm_cross_floor = 0.16 * W_TM_scatter;
@@ -194,12 +217,12 @@ __device__ __forceinline__ void llb_step(double m[6], double te_current, double 
 
 // update_Te_energy + update_4tm (Thermal.jl:24-40)
 __device__ __forceinline__ void update_4tm(double &te, double &tl, double &tsT, double &tsR, double pabs,
-                                           double QT, double QR, const mp_ferri_params &gd, double dt) {
+                                           double QT, double QR, const FilmConsts &gd, double dt) {
     const double tau_th = 2.0e-12;
     const double te0 = te, tl0 = tl, t0 = tsT, r0 = tsR;
-    const double ue = 0.5 * gd.gamma_e * (te0 * te0);
+    const double ue = gd.half_gamma_e * (te0 * te0);
     const double q = -gd.Gel * (te0 - tl0) - gd.Ges_TM * (te0 - t0) - gd.Ges_RE * (te0 - r0) + pabs;
-    const double ue_new = jl_max(0.5 * gd.gamma_e, ue + dt * q);
+    const double ue_new = jl_max(gd.half_gamma_e, ue + dt * q);
     te = sqrt(2.0 * ue_new / gd.gamma_e);
     tl = tl0 + dt * (gd.Gel * (te0 - tl0) * gd.inv_Cl) - dt * (tl0 - gd.T0) / tau_th;
     tsT = jl_max(1.0, t0 + dt * (gd.Ges_TM * (te0 - t0) - QT) * gd.inv_Cs_TM);

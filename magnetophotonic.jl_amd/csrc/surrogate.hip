@@ -59,7 +59,7 @@ __device__ __forceinline__ float sigm(float z) {
 // Raw features of one cell (the order of mp_mlp in mphip.h).
 __device__ __forceinline__ void cell_features(float f[MP_MLP_IN], double te, double tl, double tsT,
                                               double tsR, double mT, double mR, double pabs, double dt,
-                                              const mp_ferri_params &gd) {
+                                              const FilmConsts &gd) {
     const double pn = pabs * 1e-18;
     const double apn = __builtin_fabs(pn);
     const double ash = __builtin_copysign(log(apn + sqrt(apn * apn + 1.0)), pn);
@@ -79,7 +79,7 @@ __device__ __forceinline__ void cell_features(float f[MP_MLP_IN], double te, dou
 // One plain substep of the film integrator (an iteration of the multiphysics_kernel loop).
 template <typename L>
 __device__ __forceinline__ void plain_substep(double &te, double &tl, double &tsT, double &tsR, double m[6],
-                                              double pabs, const mp_ferri_params &gd, const L *tm_lut,
+                                              double pabs, const FilmConsts &gd, const L *tm_lut,
                                               const L *re_lut, double T_min, double inv_dT, int32_t lut_N,
                                               double sub_dt, int iters) {
     double out[4];
@@ -456,9 +456,10 @@ __global__ __launch_bounds__(128) void dataset_kernel(
     const L *__restrict__ Te, const L *__restrict__ Tl, const L *__restrict__ TsT, const L *__restrict__ TsR,
     const L *__restrict__ mTx, const L *__restrict__ mTy, const L *__restrict__ mTz, const L *__restrict__ mRx,
     const L *__restrict__ mRy, const L *__restrict__ mRz, const double *__restrict__ window,
-    const mp_ferri_params gd, const L *__restrict__ tm_lut, const L *__restrict__ re_lut, double T_min,
+    const FilmConsts *__restrict__ gdp, const L *__restrict__ tm_lut, const L *__restrict__ re_lut, double T_min,
     double inv_dT, int32_t lut_N, double dt, int subcycles, int iters, int64_t stride, int64_t rows,
     float *__restrict__ X, float *__restrict__ T) {
+    const FilmConsts &gd = *gdp;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const int64_t n = r * stride;
@@ -498,9 +499,10 @@ template <typename L>
 __global__ __launch_bounds__(128) void surrogate_fallback_kernel(
     L *__restrict__ Te, L *__restrict__ Tl, L *__restrict__ TsT, L *__restrict__ TsR, L *__restrict__ mTx,
     L *__restrict__ mTy, L *__restrict__ mTz, L *__restrict__ mRx, L *__restrict__ mRy, L *__restrict__ mRz,
-    const FallbackRec *__restrict__ recs, const unsigned long long *__restrict__ nrec, const mp_ferri_params gd,
+    const FallbackRec *__restrict__ recs, const unsigned long long *__restrict__ nrec, const FilmConsts *__restrict__ gdp,
     const L *__restrict__ tm_lut, const L *__restrict__ re_lut, double T_min, double inv_dT, int32_t lut_N,
     double sub_dt, int subcycles, int iters) {
+    const FilmConsts &gd = *gdp;
     const unsigned long long cnt = *nrec;
     for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < cnt;
          i += (unsigned long long)gridDim.x * blockDim.x) {
@@ -521,11 +523,12 @@ template <typename L, bool HAS_PABS>
 __global__ __launch_bounds__(256) void surrogate_kernel(
     L *__restrict__ Te, L *__restrict__ Tl, L *__restrict__ TsT, L *__restrict__ TsR, L *__restrict__ mTx,
     L *__restrict__ mTy, L *__restrict__ mTz, L *__restrict__ mRx, L *__restrict__ mRy, L *__restrict__ mRz,
-    double *__restrict__ window, const mp_ferri_params gd, const L *__restrict__ tm_lut,
+    double *__restrict__ window, const FilmConsts *__restrict__ gdp, const L *__restrict__ tm_lut,
     const L *__restrict__ re_lut, double T_min, double inv_dT, int32_t lut_N, double dt, int subcycles,
     int iters, int64_t N, const uint2 *__restrict__ packed, const float *__restrict__ P,
     const float *__restrict__ norm, double tol, unsigned long long *__restrict__ fallbacks,
     FallbackRec *__restrict__ recs, unsigned long long *__restrict__ nrec) {
+    const FilmConsts &gd = *gdp;
     __shared__ InferLds s;
     __shared__ __align__(16) uint16_t xs[4][64][16];
     __shared__ float ys[4][64][8];
@@ -641,10 +644,12 @@ template <typename L>
 static int launch_dataset(const mp_film *f, const mp_ferri_params *p, const mp_lut *lut, double dt, int subcycles,
                           int iters, int has_pabs, int64_t stride, float *X, float *T, int64_t rows) {
     const double dtc = (double)(L)dt, tmin = (double)(L)lut->T_min, invdt = (double)(L)lut->inv_dT;
+    const FilmConsts *gc = film_consts(p);
+    if (!gc) return MP_ERR_HIP;
     KernelTimer t("surrogate_dataset");
 #define DS_ARGS (const L *)f->Te, (const L *)f->Tl, (const L *)f->Ts_TM, (const L *)f->Ts_RE, (const L *)f->m_TM_x, \
                 (const L *)f->m_TM_y, (const L *)f->m_TM_z, (const L *)f->m_RE_x, (const L *)f->m_RE_y,         \
-                (const L *)f->m_RE_z, f->window_sum, *p, (const L *)lut->tm, (const L *)lut->re, tmin, invdt,   \
+                (const L *)f->m_RE_z, f->window_sum, gc, (const L *)lut->tm, (const L *)lut->re, tmin, invdt,   \
                 lut->n, dtc, subcycles, iters, stride, rows, X, T
     if (has_pabs) dataset_kernel<L, true><<<cdiv(rows, 128), 128, 0, stream()>>>(DS_ARGS);
     else dataset_kernel<L, false><<<cdiv(rows, 128), 128, 0, stream()>>>(DS_ARGS);
@@ -675,11 +680,13 @@ static int launch_surrogate(mp_film *f, const mp_ferri_params *p, const mp_lut *
     const int64_t N = f->n;
     const double dtc = (double)(L)dt, tmin = (double)(L)lut->T_min, invdt = (double)(L)lut->inv_dT;
     if (int e = ensure_fallback_list(N)) return e;
+    const FilmConsts *gc = film_consts(p);
+    if (!gc) return MP_ERR_HIP;
     reset_count_kernel<<<1, 1, 0, stream()>>>(g_nrec);
     MP_LAUNCH_CHECK("reset_count_kernel");
     KernelTimer t("surrogate");
 #define SG_ARGS (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, \
-                (L *)f->m_TM_z, (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, f->window_sum, *p,    \
+                (L *)f->m_TM_z, (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, f->window_sum, gc,    \
                 (const L *)lut->tm, (const L *)lut->re, tmin, invdt, lut->n, dtc, subcycles, iters, N, \
                 (const uint2 *)m->packed, m->params, m->norm, tol, (unsigned long long *)fallbacks, g_recs, g_nrec
     if (has_pabs) surrogate_kernel<L, true><<<cdiv(N, 256), 256, 0, stream()>>>(SG_ARGS);
@@ -693,7 +700,7 @@ static int launch_surrogate(mp_film *f, const mp_ferri_params *p, const mp_lut *
         const int64_t blocks = std::min<int64_t>(cdiv(N, 128), 4096);
         surrogate_fallback_kernel<L><<<blocks, 128, 0, stream()>>>(
             (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, (L *)f->m_TM_z,
-            (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, g_recs, g_nrec, *p, (const L *)lut->tm,
+            (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, g_recs, g_nrec, gc, (const L *)lut->tm,
             (const L *)lut->re, tmin, invdt, lut->n, dtc / (double)subcycles, subcycles, iters);
         MP_LAUNCH_CHECK("surrogate_fallback_kernel");
     }
