@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "magnetophotonic.jl_amd"))
 
 METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Δm|≤1e-3"
 HBM_PEAK_GBS = 8000.0
-KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "multiphysics",
+KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "film_epilogue", "multiphysics",
            "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate", "dft_snapshot", "dft_batch")
 DFT_BATCH = int(os.environ.get("MP_DFT_BATCH", "32"))
 
@@ -173,7 +173,7 @@ def main():
     ncell = nx * ny * nz
     mcell = ncell * args.steps / wall / 1e6
     eps_idx = st.yee.n_eps > 0 and os.environ.get("MP_YEE_NO_EPS_INDEX") is None
-    kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH)
+    kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH, st.fused_epilogue)
     if "multiphysics" in kb:
         kb["surrogate"] = kb["multiphysics"]
     # the dominant kernel is the largest HBM consumer of the step (algorithmic bytes);
@@ -204,6 +204,7 @@ def main():
                    "form": args.form, "dft_planes": len(planes), "dft_bins": len(bins),
                    "llb_substeps": spec.subcycles, "cpml": list(spec.cpml.widths),
                    "inv_eps": f"uint8 index, {st.yee.n_eps} triples" if eps_idx else "FP32 volumes",
+                   "film_list_work": "one fused pass" if st.fused_epilogue else "five launches",
                    "dft_batch": DFT_BATCH},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -264,6 +264,11 @@ typedef struct mp_state {
     const mp_mlp *surrogate;
     double surrogate_tol;
     int64_t *surrogate_fallbacks;
+    /* Non-zero when every ADE list and the MO list are the film list (entry n is film
+     * cell n) and the ADE state is stored in the compute type: the step then runs the
+     * list work after the E update as one mp_film_epilogue pass. */
+    int32_t film_lists_coincide;
+    int32_t _pad_state;
 } mp_state;
 
 /* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */
@@ -352,6 +357,23 @@ int mp_patch_E_mo_gyration(void *Ea, void *Eb, mp_mo *s, const void *inv_eps_a,
  * has_ade_work == 0.  Replaces _ka_pabs_accumulate! (Kernels.jl:623-646; :313-355). */
 int mp_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_ade *ay,
                        const mp_ade *az, int32_t has_ade_work, double pabs_factor, double dt);
+
+/* The step's film list work after the E update in one pass over the film cells: the ADE
+ * patch of Ex, Ey and Ez, the gyrotropic exchange of the pair (mo_ea, mo_eb) and the
+ * absorbed-power accumulation, each cell in the order of _step_typed!
+ * (src/fdtd/Solver.jl:310-345).  The result is that of mp_patch_E_dispersive on x, y, z,
+ * then mp_patch_E_mo_gyration, then mp_pabs_accumulate, bit for bit.  It requires every
+ * list to be the film list: entry n of each ADE list and of the MO list is film cell n,
+ * as FDTDState builds them from the material cells (Solver.jl:114-159); the caller
+ * vouches for that (the lengths are checked).  ADE state must be stored in the compute
+ * type; mo may be NULL (no gyration).  mo_ea / mo_eb are two of g's E volumes and
+ * mo_iea / mo_ieb their inv_eps volumes.  Replaces the launch sequence of
+ * _ka_patch_E_dispersive! x3, _ka_patch_E_mo_gyration! and _ka_pabs_accumulate!
+ * (Kernels.jl:595-646) when the lists coincide. */
+int mp_film_epilogue(const mp_yee *g, mp_ade *ax, mp_ade *ay, mp_ade *az, mp_mo *mo, void *mo_ea,
+                     void *mo_eb, const void *mo_iea, const void *mo_ieb, mp_film *f,
+                     int32_t has_ade_work, double pabs_factor, double Q_voigt_TM, double Q_voigt_RE,
+                     double dt);
 
 /* 4TM + two-sublattice LLB, `subcycles` substeps of dt/subcycles, FP64 arithmetic.
  * has_pabs = 1 consumes window_sum/dt and resets it; 0 is the cool-down form

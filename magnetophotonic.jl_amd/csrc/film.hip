@@ -203,6 +203,186 @@ static int launch_pabs(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_a
     return MP_OK;
 }
 
+
+// ---- fused film epilogue ------------------------------------------------------------------
+// The step's list work after the E update in one pass over the film cells: the ADE patch of
+// Ex, Ey and Ez, then the gyrotropic exchange of the pair (a, b), then the absorbed power.
+// Every list is the film list (entry n of each ADE list, of the MO list and of the film is
+// the same cell), so each cell runs exactly the operations of ade_patch_kernel x3,
+// mo_gyration_kernel and pabs_kernel, in that order, on its own values: the result is that
+// of the separate kernels bit for bit.  E is read and written once per cell instead of by
+// every kernel, J stays in registers for the absorbed power, and no position map is read.
+template <typename C> struct AdeView {
+    C *P, *J, *E_old;
+    const C *fill, *inv_eps;
+};
+template <typename C> struct MoView {
+    C *Pa, *Ja, *Pb, *Jb, *Eoa, *Eob;
+    const C *fill, *m_TM, *m_RE;
+};
+
+// ade_patch_kernel on one entry whose E is held in `Ecell` (storage type S).  jsum is the sum
+// over poles of J as stored after the call, in the order pabs_kernel adds it.
+template <typename S, typename C>
+__device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C> &a, const C *__restrict__ poles, int np,
+                                          int64_t n, int64_t N, C inv_alpha, double &jsum) {
+    const C Eprev = a.E_old[n];
+    const C f = a.fill[n];
+    const C ie = a.inv_eps[n];
+    C Pold[MAXP], Jold[MAXP];
+    C sum_psi = (C)0, sum_beta = (C)0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+        if (p >= np) break;
+        Pold[p] = a.P[n + p * N];
+        Jold[p] = a.J[n + p * N];
+        const C beta = poles[3 * p + 2] * f;
+        const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
+        sum_psi += psi;
+        sum_beta += beta;
+    }
+    const C den = (C)1 + sum_beta * ie;
+    const C Etemp = (C)Ecell;
+    jsum = 0.0;
+    if (finiteT(Etemp) && finiteT(sum_psi) && finiteT(den) && __builtin_fabs((double)den) >= (double)(C)1e-30) {
+        const C Efinal = (Etemp - sum_psi * ie) / den;
+        Ecell = (S)Efinal;
+        a.E_old[n] = Efinal;
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+            if (p >= np) break;
+            const C beta = poles[3 * p + 2] * f;
+            const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
+            const C Pnew = psi + beta * Efinal;
+            const C Jnew = (Pnew - Pold[p]) * inv_alpha - Jold[p];
+            a.P[n + p * N] = Pnew;
+            a.J[n + p * N] = Jnew;
+            jsum += (double)Jnew;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+            if (p >= np) break;
+            jsum += (double)Jold[p];
+        }
+    }
+}
+
+template <typename S, typename C, bool HAS_MO, bool ADE_WORK>
+__global__ __launch_bounds__(256) void film_epilogue_kernel(
+    S *__restrict__ Ex, S *__restrict__ Ey, S *__restrict__ Ez, const int32_t *__restrict__ cells,
+    AdeView<C> ax, AdeView<C> ay, AdeView<C> az, const C *__restrict__ poles, int np, C inv_alpha,
+    MoView<C> mo, const C *__restrict__ mo_poles, int mo_np, int mo_a, int mo_b,
+    const S *__restrict__ iea, const S *__restrict__ ieb, C Q_TM, C Q_RE, C mo_inv_alpha,
+    double *__restrict__ U_abs, double *__restrict__ window, double *__restrict__ peak, double pabs_factor,
+    double dt, int64_t N) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const int64_t idx = cells[n];
+    S e[3] = {Ex[idx], Ey[idx], Ez[idx]};
+    double js[3];
+    ade_entry<S, C>(e[0], ax, poles, np, n, N, inv_alpha, js[0]);
+    ade_entry<S, C>(e[1], ay, poles, np, n, N, inv_alpha, js[1]);
+    ade_entry<S, C>(e[2], az, poles, np, n, N, inv_alpha, js[2]);
+    if (HAS_MO) {
+        // mo_gyration_kernel on entry n (its film position is n)
+        const C f = mo.fill[n];
+        const C qeff = Q_TM * mo.m_TM[n] + Q_RE * mo.m_RE[n];
+        const S ea_s = mo_a == 0 ? e[0] : (mo_a == 1 ? e[1] : e[2]);
+        const S eb_s = mo_b == 0 ? e[0] : (mo_b == 1 ? e[1] : e[2]);
+        const C Ea_drive = (C)ea_s;
+        const C Eb_drive = (C)eb_s;
+        const C Ea_old = mo.Eoa[n];
+        const C Eb_old = mo.Eob[n];
+        C sum_psi_a = (C)0, sum_psi_b = (C)0, sumC3 = (C)0;
+        for (int p = 0; p < mo_np; ++p) {
+            const C C1 = mo_poles[3 * p], C2 = mo_poles[3 * p + 1], C3 = mo_poles[3 * p + 2];
+            const C beta = C3 * f * qeff;
+            sum_psi_a += fmaT<C>(C1, mo.Pa[n + p * N], fmaT<C>(C2, mo.Ja[n + p * N], beta * Eb_old));
+            sum_psi_b += fmaT<C>(C1, mo.Pb[n + p * N], fmaT<C>(C2, mo.Jb[n + p * N], beta * Ea_old));
+            sumC3 += C3;
+        }
+        const C ica = (C)iea[idx];
+        const C icb = (C)ieb[idx];
+        const C Bmo = f * qeff * sumC3;
+        const C half = (C)0.5;
+        const C aa = half * ica * Bmo;
+        const C ab = half * icb * Bmo;
+        const C RHSa = Ea_drive - ica * sum_psi_a - aa * Eb_drive;
+        const C RHSb = Eb_drive + icb * sum_psi_b + ab * Ea_drive;
+        const C den = (C)1 + aa * ab;
+        const C Eb_final = (RHSb + ab * RHSa) / den;
+        const C Ea_final = RHSa - aa * Eb_final;
+        if (finiteT(Ea_final) && finiteT(Eb_final)) {
+            const S ea_new = (S)Ea_final, eb_new = (S)Eb_final;
+            if (mo_a == 0) e[0] = ea_new; else if (mo_a == 1) e[1] = ea_new; else e[2] = ea_new;
+            if (mo_b == 0) e[0] = eb_new; else if (mo_b == 1) e[1] = eb_new; else e[2] = eb_new;
+            mo.Eoa[n] = Ea_final;
+            mo.Eob[n] = Eb_final;
+            const C Ea_mid = half * (Ea_drive + Ea_final);
+            const C Eb_mid = half * (Eb_drive + Eb_final);
+            for (int p = 0; p < mo_np; ++p) {
+                const C C1 = mo_poles[3 * p], C2 = mo_poles[3 * p + 1], C3 = mo_poles[3 * p + 2];
+                const C beta = C3 * f * qeff;
+                const C Pa_old = mo.Pa[n + p * N], Ja_old = mo.Ja[n + p * N];
+                const C Pa_new = fmaT<C>(C1, Pa_old, fmaT<C>(C2, Ja_old, beta * Eb_old)) + beta * Eb_mid;
+                mo.Pa[n + p * N] = Pa_new;
+                mo.Ja[n + p * N] = (Pa_new - Pa_old) * mo_inv_alpha - Ja_old;
+                const C Pb_old = mo.Pb[n + p * N], Jb_old = mo.Jb[n + p * N];
+                const C Pb_new = fmaT<C>(C1, Pb_old, fmaT<C>(C2, Jb_old, beta * Ea_old)) + beta * Ea_mid;
+                mo.Pb[n + p * N] = Pb_new;
+                mo.Jb[n + p * N] = (Pb_new - Pb_old) * mo_inv_alpha - Jb_old;
+            }
+        }
+    }
+    Ex[idx] = e[0];
+    Ey[idx] = e[1];
+    Ez[idx] = e[2];
+    // pabs_kernel on film cell n
+    const double ex = (double)e[0], ey = (double)e[1], ez = (double)e[2];
+    const double pabs = ADE_WORK ? jl_max(ex * js[0] + ey * js[1] + ez * js[2], 0.0)
+                                 : pabs_factor * (ex * ex + ey * ey + ez * ez);
+    const double pdt = pabs * dt;
+    U_abs[n] += pdt;
+    window[n] += pdt;
+    if (pabs > peak[n]) peak[n] = pabs;
+}
+
+template <typename S, typename C>
+static int launch_epilogue(const mp_yee *g, mp_ade *const ade[3], const mp_mo *mo, int mo_a, int mo_b,
+                           const void *iea, const void *ieb, mp_film *f, int has_ade_work, double pabs_factor,
+                           double Q_TM, double Q_RE, double dt) {
+    const int64_t N = f->n;
+    AdeView<C> v[3];
+    for (int c = 0; c < 3; ++c)
+        v[c] = AdeView<C>{(C *)ade[c]->P, (C *)ade[c]->J, (C *)ade[c]->E_old, (const C *)ade[c]->fill,
+                          (const C *)ade[c]->inv_eps};
+    MoView<C> m{};
+    if (mo)
+        m = MoView<C>{(C *)mo->Pa_from_b, (C *)mo->Ja_from_b, (C *)mo->Pb_from_a, (C *)mo->Jb_from_a,
+                      (C *)mo->E_old_a, (C *)mo->E_old_b, (const C *)mo->fill, (const C *)f->m_TM_x,
+                      (const C *)f->m_RE_x};
+    // the separate launchers form these as CT(2)/CT(dt) (ADE) and 2/CT(dt) (MO): the same value
+    const C inv_alpha = (C)2 / (C)dt;
+    const C mo_inv_alpha = sizeof(C) == 4 ? (C)(2.0f / (float)dt) : (C)(2.0 / dt);
+    KernelTimer t("film_epilogue");
+#define EP_ARGS (S *)g->ex, (S *)g->ey, (S *)g->ez, f->cells, v[0], v[1], v[2], (const C *)ade[0]->poles,       \
+                ade[0]->npoles, inv_alpha, m, mo ? (const C *)mo->poles : nullptr, mo ? mo->npoles : 0, mo_a, mo_b, \
+                (const S *)iea, (const S *)ieb, (C)Q_TM, (C)Q_RE, mo_inv_alpha, f->U_abs, f->window_sum, f->peak, \
+                pabs_factor, dt, N
+    const unsigned grid = cdiv(N, 256);
+    if (mo) {
+        if (has_ade_work) film_epilogue_kernel<S, C, true, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        else film_epilogue_kernel<S, C, true, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
+    } else {
+        if (has_ade_work) film_epilogue_kernel<S, C, false, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        else film_epilogue_kernel<S, C, false, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
+    }
+#undef EP_ARGS
+    MP_LAUNCH_CHECK("film_epilogue_kernel");
+    return MP_OK;
+}
+
 }  // namespace mp
 
 using namespace mp;
@@ -292,4 +472,54 @@ MP_API int mp_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, con
         return launch_pabs<float, double, double>(f, g, ax, ay, az, 1, pabs_factor, dt);
     }
     return fail_arg("mp_pabs_accumulate: unsupported dtypes");
+}
+
+static int component_of(const mp_yee *g, const void *E) {
+    return E == g->ex ? 0 : (E == g->ey ? 1 : (E == g->ez ? 2 : -1));
+}
+
+MP_API int mp_film_epilogue(const mp_yee *g, mp_ade *ax, mp_ade *ay, mp_ade *az, mp_mo *mo, void *mo_ea,
+                            void *mo_eb, const void *mo_iea, const void *mo_ieb, mp_film *f, int32_t has_ade_work,
+                            double pabs_factor, double Q_TM, double Q_RE, double dt) {
+    if (!g || !ax || !ay || !az || !f) return fail_arg("mp_film_epilogue: null argument");
+    if (f->n <= 0) return MP_OK;
+    mp_ade *ade[3] = {ax, ay, az};
+    const int64_t N = f->n;
+    for (int c = 0; c < 3; ++c) {
+        const mp_ade *a = ade[c];
+        if (a->n != N) return fail_arg("mp_film_epilogue: ADE list %d has %lld entries, the film %lld", c,
+                                       (long long)a->n, (long long)N);
+        if (a->npoles != ax->npoles) return fail_arg("mp_film_epilogue: ADE lists disagree on npoles");
+        if (a->npoles < 1 || a->npoles > MAXP) return fail_arg("mp_film_epilogue: 1..%d poles", MAXP);
+        if (a->dtype != ax->dtype || a->state_dtype != a->dtype)
+            return fail_arg("mp_film_epilogue: ADE state must be stored in the compute type");
+        if (!a->P || !a->J || !a->E_old || !a->fill || !a->inv_eps || !a->poles)
+            return fail_arg("mp_film_epilogue: null ADE array");
+    }
+    if (!f->cells || !f->U_abs || !f->window_sum || !f->peak) return fail_arg("mp_film_epilogue: null film array");
+    int mo_a = -1, mo_b = -1;
+    if (mo && mo->n > 0 && mo->npoles > 0) {
+        if (mo->n != N) return fail_arg("mp_film_epilogue: MO list must be the film list");
+        if (mo->dtype != ax->dtype) return fail_arg("mp_film_epilogue: MO dtype must equal the ADE compute type");
+        mo_a = component_of(g, mo_ea);
+        mo_b = component_of(g, mo_eb);
+        if (mo_a < 0 || mo_b < 0 || mo_a == mo_b)
+            return fail_arg("mp_film_epilogue: the MO pair must be two distinct E volumes of the state");
+        if (!mo_iea || !mo_ieb || !mo->Pa_from_b || !mo->Ja_from_b || !mo->Pb_from_a || !mo->Jb_from_a ||
+            !mo->E_old_a || !mo->E_old_b || !mo->fill || !mo->poles || !f->m_TM_x || !f->m_RE_x)
+            return fail_arg("mp_film_epilogue: null MO array");
+    } else {
+        mo = nullptr;
+    }
+    const int32_t fd = g->field_dtype, ct = ax->dtype;
+    if (fd == MP_F32 && ct == MP_F32)
+        return launch_epilogue<float, float>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
+                                             Q_TM, Q_RE, dt);
+    if (fd == MP_F64 && ct == MP_F64)
+        return launch_epilogue<double, double>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
+                                               Q_TM, Q_RE, dt);
+    if (fd == MP_F32 && ct == MP_F64)
+        return launch_epilogue<float, double>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
+                                              Q_TM, Q_RE, dt);
+    return fail_arg("mp_film_epilogue: unsupported dtypes field=%d ct=%d", fd, ct);
 }

@@ -116,12 +116,38 @@ static int launch_deferred(mp_state *s) {
     return launch_film_side(s, g_ov.deferred_dt);
 }
 
+static int film_list_work(mp_state *s);
+static int film_update_due(mp_state *s);
+
 static int one_step(mp_state *s) {
     int rc;
     if ((rc = mp_update_H_3d(s->yee, s->cpml, s->dt, s->mu0))) return rc;
     if ((rc = launch_deferred(s))) return rc;   // the previous step's film update, beside this E update
     if ((rc = inject(s))) return rc;
     if ((rc = mp_update_E_3d(s->yee, s->cpml, s->dt))) return rc;
+    if (s->film_lists_coincide && s->ade[0] && s->film) {
+        // the film list work in one pass; it contains the gyration, which reads the film
+        // update of the previous step
+        if ((rc = overlap_join())) return rc;
+        if ((rc = mp_film_epilogue(s->yee, s->ade[0], s->ade[1], s->ade[2], s->mo, s->mo_ea, s->mo_eb, s->mo_iea,
+                                   s->mo_ieb, s->film, s->has_ade_work, s->pabs_factor, s->Q_TM, s->Q_RE, s->dt)))
+            return rc;
+        s->n += 1;
+        s->t += s->dt;
+        if ((rc = film_update_due(s))) return rc;
+    } else {
+        if ((rc = film_list_work(s))) return rc;
+    }
+    if (s->record_dft && s->nplanes > 0) {
+        if ((rc = dft_record_batched(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t, s->dt)))
+            return rc;
+    }
+    return MP_OK;
+}
+
+// The list kernels after the E update, one launch each (the lists need not coincide).
+static int film_list_work(mp_state *s) {
+    int rc;
     if (s->ade[0]) {
         void *E[3] = {s->yee->ex, s->yee->ey, s->yee->ez};
         for (int c = 0; c < 3; ++c)
@@ -140,6 +166,15 @@ static int one_step(mp_state *s) {
         if ((rc = mp_pabs_accumulate(s->film, s->yee, s->ade[0], s->ade[1], s->ade[2], s->has_ade_work,
                                      s->pabs_factor, s->dt)))
             return rc;
+        if ((rc = film_update_due(s))) return rc;
+    }
+    return MP_OK;
+}
+
+// The 4TM + LLB update when the multiphysics window closes at step n (Solver.jl:356-367).
+static int film_update_due(mp_state *s) {
+    int rc;
+    if (s->film) {
         if (s->multiphysics_enabled && s->multiphysics_every > 0 && s->n % s->multiphysics_every == 0) {
             const int64_t window = s->n - s->last_mp_n;
             if (window > 0) {
@@ -155,10 +190,6 @@ static int one_step(mp_state *s) {
                 s->last_mp_n = s->n;
             }
         }
-    }
-    if (s->record_dft && s->nplanes > 0) {
-        if ((rc = dft_record_batched(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t, s->dt)))
-            return rc;
     }
     return MP_OK;
 }

@@ -90,7 +90,8 @@ class State(C.Structure):
                 ("Q_RE", f64), ("t", f64), ("n", i64), ("last_mp_n", i64),
                 ("multiphysics_every", i32), ("subcycles", i32), ("brillouin_iters", i32),
                 ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32),
-                ("surrogate", C.POINTER(Mlp)), ("surrogate_tol", f64), ("surrogate_fallbacks", vp)]
+                ("surrogate", C.POINTER(Mlp)), ("surrogate_tol", f64), ("surrogate_fallbacks", vp),
+                ("film_lists_coincide", i32), ("_pad_state", i32)]
 
 
 MP_PULSE_GAUSSIAN, MP_PULSE_CARRIER_GAUSSIAN, MP_PULSE_CONTINUOUS = 1, 2, 3
@@ -106,6 +107,8 @@ SIGNATURES = {
     "patch_E_dispersive": (C.c_int, [vp, P(Ade), i32, f64]),
     "patch_E_mo_gyration": (C.c_int, [vp, vp, P(Mo), vp, vp, i32, vp, vp, f64, f64, f64]),
     "pabs_accumulate": (C.c_int, [P(Film), P(Yee), P(Ade), P(Ade), P(Ade), i32, f64, f64]),
+    "film_epilogue": (C.c_int, [P(Yee), P(Ade), P(Ade), P(Ade), P(Mo), vp, vp, vp, vp, P(Film), i32, f64, f64,
+                                f64, f64]),
     "multiphysics_step": (C.c_int, [P(Film), vp, P(Lut), f64, i32, i32, i32]),
     "probe_dft_accumulate": (C.c_int, [P(Yee), P(Plane), i32, vp, vp, i32, f64, f64]),
     "probe_trace_plane": (C.c_int, [P(Yee), i32, i64, i64, vp, vp, vp]),

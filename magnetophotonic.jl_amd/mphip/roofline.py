@@ -29,9 +29,11 @@ def _psi_cells(spec) -> int:
     return tot
 
 
-def kernel_bytes(spec, eps_index: bool = False) -> dict:
+def kernel_bytes(spec, eps_index: bool = False, fused_epilogue: bool = False) -> dict:
     """Algorithmic bytes per launch of each kernel of one step.  eps_index: the E update
-    reads inv_eps as a 1-byte material index per cell instead of three volumes."""
+    reads inv_eps as a 1-byte material index per cell instead of three volumes.
+    fused_epilogue: the ADE, gyration and absorbed-power list work runs as one pass over
+    the film cells (E read and written once, J summed in registers, no position maps)."""
     nx, ny, nz = spec.shape
     fs = spec.field_dtype.itemsize
     cs = spec.compute_dtype.itemsize
@@ -57,6 +59,14 @@ def kernel_bytes(spec, eps_index: bool = False) -> dict:
         work = (12 + 3 * np_ * ss) if spec.has_ade_work() else 0
         out["pabs"] = nf * (4 + 3 * fs + work + 6 * 8)
         out["multiphysics"] = nf * (20 * cs + 2 * 8)
+        if fused_epilogue and spec.ade is not None:
+            per = 4 + 6 * fs + 3 * (4 * cs + 4 * np_ * cs) + 6 * 8   # cell, E r/w, 3 ADE entries, U/window/peak
+            if spec.mo is not None:
+                npm = len(spec.mo_poles) // 3
+                per += 3 * cs + 4 * cs + 8 * npm * cs + 2 * fs       # fill, m x2, E_old r/w x2, P/J, inv_eps x2
+            for k in ("ade_patch", "mo_gyration", "pabs"):
+                out.pop(k, None)
+            out["film_epilogue"] = per * nf
     return out
 
 
@@ -75,11 +85,12 @@ def plane_cells(spec, planes) -> int:
     return sum(dims[a] for a, _ in planes)
 
 
-def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch: int = 1) -> dict:
+def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch: int = 1,
+               fused_epilogue: bool = False) -> dict:
     """Algorithmic bytes per step.  With dft_batch > 1 the DFT record is a per-step
     snapshot (4 fields read, 4 written) plus, once per batch, the stored steps read and
     the accumulators read and written once."""
-    kb = kernel_bytes(spec, eps_index)
+    kb = kernel_bytes(spec, eps_index, fused_epilogue)
     if planes and nb:
         if dft_batch > 1:
             fs = spec.field_dtype.itemsize

@@ -337,6 +337,39 @@ class FDTDState:
         st.multiphysics_every, st.subcycles = spec.multiphysics_every, spec.subcycles
         st.brillouin_iters = spec.brillouin_iters
         st.has_ade_work = 1 if spec.has_ade_work() else 0
+        st.film_lists_coincide = 1 if self._lists_coincide() else 0
+
+    def _lists_coincide(self) -> bool:
+        """Every ADE list and the MO list is the film list (entry n is film cell n) and the
+        ADE state is stored in the compute type: the step may run its list work as one
+        mp_film_epilogue pass (the lists FDTDState builds from the material cells,
+        Solver.jl:114-159)."""
+        spec = self.spec
+        if self.film is None or spec.ade is None or spec.n_film == 0:
+            return False
+        if _PSI_CODE[spec.ade_state_dtype] != _DT_CODE[np.dtype(spec.compute_dtype)]:
+            return False
+        ident = np.arange(spec.n_film, dtype=np.int64)
+        for c in range(3):
+            if not np.array_equal(spec.ade[c].idx, spec.film_cells):
+                return False
+            if not np.array_equal(spec.pos[c], ident):
+                return False
+        if spec.mo is not None and spec.mo_pos is not None:
+            if not np.array_equal(spec.mo.idx, spec.film_cells) or not np.array_equal(spec.mo_pos, ident):
+                return False
+        return True
+
+    @property
+    def fused_epilogue(self) -> bool:
+        """Whether the step runs its film list work as one mp_film_epilogue pass."""
+        return bool(self._st.film_lists_coincide)
+
+    @fused_epilogue.setter
+    def fused_epilogue(self, on: bool) -> None:
+        if on and not self._lists_coincide():
+            raise ValueError("the ADE and MO lists are not the film list; the fused pass does not apply")
+        self._st.film_lists_coincide = 1 if on else 0
 
     def set_source(self, src) -> None:
         """Replace the state's source (the phase-source swap of run_phase!,

@@ -369,6 +369,28 @@ ORC_EXPORT int orc_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax
     return 0;
 }
 
+/* mp_film_epilogue: the film list work after the E update, launched as the reference
+ * launches it (_step_typed!, src/fdtd/Solver.jl:310-345): the ADE patch of Ex, Ey, Ez
+ * (Kernels.jl:595-604), the gyration (:606-619), the absorbed power (:623-646). */
+ORC_EXPORT int orc_film_epilogue(const mp_yee *g, mp_ade *ax, mp_ade *ay, mp_ade *az, mp_mo *mo,
+                                 void *mo_ea, void *mo_eb, const void *mo_iea, const void *mo_ieb,
+                                 mp_film *f, int32_t has_ade_work, double pabs_factor, double Q_TM,
+                                 double Q_RE, double dt) {
+    if (f->n <= 0) return 0;
+    void *E[3] = {g->ex, g->ey, g->ez};
+    mp_ade *a[3] = {ax, ay, az};
+    for (int c = 0; c < 3; ++c) {
+        const int rc = orc_patch_E_dispersive(E[c], a[c], g->field_dtype, dt);
+        if (rc) return rc;
+    }
+    if (mo && mo->n > 0) {
+        const int rc = orc_patch_E_mo_gyration(mo_ea, mo_eb, mo, mo_iea, mo_ieb, g->field_dtype, f->m_TM_x,
+                                               f->m_RE_x, Q_TM, Q_RE, dt);
+        if (rc) return rc;
+    }
+    return orc_pabs_accumulate(f, g, ax, ay, az, has_ade_work, pabs_factor, dt);
+}
+
 ORC_EXPORT int orc_multiphysics_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
                                      double dt, int32_t subcycles, int32_t brillouin_iters,
                                      int32_t has_pabs) {

@@ -293,3 +293,31 @@ def test_batched_dft_record_is_bitwise_per_step():
         assert np.array_equal(a.field(name), b.field(name)), name
     a.free()
     b.free()
+
+
+def test_fused_film_epilogue_matches_separate_kernels():
+    """The one-pass film list work (ADE x/y/z, gyration, absorbed power) against the five
+    separate launches on the same state: every field, list state and film value equal."""
+    from mphip import configs
+    from mphip.state import FDTDState
+    spec, meta = configs.coupled_pump(nx=40, ny=36, nz=64, steps=150, fluence_mj_cm2=12.0, npml=6,
+                                       src_z=40e-9)
+    print("case coupled_pump 40x36x64 fused vs separate, 150 steps", flush=True)
+    fused, sep = FDTDState(spec), FDTDState(spec)
+    assert fused.fused_epilogue, "the pump's lists are the film list"
+    sep.fused_epilogue = False
+    fused.run(150)
+    sep.run(150)
+    fused.synchronize()
+    for name in FIELDS:
+        assert np.array_equal(fused.field(name), sep.field(name)), name
+    for c in range(3):
+        a, b = fused.ade_state(c), sep.ade_state(c)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (c, k)
+    fa, fb = fused.film_state(), sep.film_state()
+    assert fb["Te"].max() > 300.5
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+    fused.free()
+    sep.free()
