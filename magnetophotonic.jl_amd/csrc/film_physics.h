@@ -26,14 +26,35 @@ struct FilmConsts {
     // brillouin(S, x) of each sublattice (0 = TM, 1 = RE): a = (2S+1)/(2S), b = 1/(2S) and
     // the small-x series coefficients (a^2 - b^2)/3 and (a^4 - b^4)/45
     double bri_a[2], bri_b[2], bri_c1[2], bri_c3[2];
+    // 2S+1 when it is an integer in [2, 64] (every physical spin), else 0
+    int32_t bri_pow[2];
 };
 
 // Device copy of the constants for `p` (cached per device; re-derived and uploaded only
 // when the parameters change, after the device has drained).  NULL on error.
 const FilmConsts *film_consts(const mp_ferri_params *p);
 
+// q^p for a uniform integer p >= 1, by squaring.
+__device__ __forceinline__ double ipow(double q, int p) {
+    double r = 1.0;
+    while (true) {
+        if (p & 1) r *= q;
+        p >>= 1;
+        if (!p) return r;
+        q *= q;
+    }
+}
+
 // brillouin(S, x) (Magnetization.jl:7-22) with a, b and the series coefficients of the
 // sublattice taken from the constants (k = 0: TM, 1: RE).
+//
+// B = a coth(ax) - b coth(bx).  The reference forms each coth as (e+1)/(e-1) from its own
+// exp(2ax) and exp(2bx).  Since a = (2S+1) b, both come from one exponential when 2S+1 is
+// an integer: with q = exp(-2|bx|) and q_a = q^(2S+1) = exp(-2|ax|),
+//   B = sign(x) [a (1+q_a)(1-q) - b (1+q)(1-q_a)] / ((1-q_a)(1-q)),
+// one exponential and one division instead of two of each, with q <= 1 so nothing
+// overflows.  Where the reference's exp(2ax) overflows (2|ax| > 709.78, |ax| <= 500) it
+// yields inf/inf = NaN; that is kept.  The results agree with the reference's to rounding.
 __device__ __forceinline__ double brillouin(const FilmConsts &c, int k, double x) {
     if (!__builtin_isfinite(x)) return 0.0;
     const double a = c.bri_a[k];
@@ -41,6 +62,16 @@ __device__ __forceinline__ double brillouin(const FilmConsts &c, int k, double x
     if (__builtin_fabs(x) < 1e-4) return c.bri_c1[k] * x - c.bri_c3[k] * (x * x * x);
     const double ax = a * x;
     const double bx = b * x;
+    const int p = c.bri_pow[k];
+    const double aax = __builtin_fabs(ax);
+    if (p > 0 && aax <= 354.89135644669199) {
+        const double q = exp(-2.0 * __builtin_fabs(bx));
+        const double qa = ipow(q, p);
+        const double num = a * ((1.0 + qa) * (1.0 - q)) - b * ((1.0 + q) * (1.0 - qa));
+        const double den = (1.0 - qa) * (1.0 - q);
+        return jl_clamp(__builtin_copysign(num / den, x), -1.0, 1.0);
+    }
+    if (p > 0 && aax <= 500.0 && ax > 0.0) return __builtin_nan("");
     const double coth_ax = __builtin_fabs(ax) > 500.0 ? __builtin_copysign(1.0, ax) : (exp(2.0 * ax) + 1.0) / (exp(2.0 * ax) - 1.0);
     const double coth_bx = __builtin_fabs(bx) > 500.0 ? __builtin_copysign(1.0, bx) : (exp(2.0 * bx) + 1.0) / (exp(2.0 * bx) - 1.0);
     return jl_clamp(a * coth_ax - b * coth_bx, -1.0, 1.0);

@@ -58,6 +58,8 @@ static FilmConsts derive_consts(const mp_ferri_params &p) {
         c.bri_b[k] = b;
         c.bri_c1[k] = (a * a - b * b) / 3.0;
         c.bri_c3[k] = (a2 * a2 - b2 * b2) / 45.0;
+        const double p = 2.0 * S[k] + 1.0;
+        c.bri_pow[k] = (p == std::floor(p) && p >= 2.0 && p <= 64.0) ? (int32_t)p : 0;
     }
     return c;
 }
