@@ -13,7 +13,10 @@
 // Arithmetic order is the reference's: derivative = (E[n] - E[c]) * inv_d, psi update
 // = fma(b, psi, a*deriv) (Julia muladd), curl terms and the D/H updates exactly as
 // written there, compiled with -ffp-contract=off.
+#include <algorithm>
 #include <cstdlib>
+#include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -234,9 +237,7 @@ constexpr int NXCD = 8;
 // Flattened plane mapping of the vector kernels: consecutive rows of a z-plane are
 // adjacent in memory, so a plane is one run of nx*ny/4 float4 groups, row after row.
 // Lanes take consecutive groups across row ends (no idle lanes when a row is not a
-// multiple of 256 cells) and every load of a wave stays contiguous.  A lane's x
-// neighbour from a shuffle may belong to the next row; it is needed only for the cell
-// the update skips there (H: i = nx-1, E: i = 0).
+// multiple of 256 cells) and every load of a wave stays contiguous.
 struct PlaneGroup { int64_t i0, j, bz; bool in_plane; };
 __device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_t bpp, int64_t nbz, bool remap,
                                                   int tyv) {
@@ -255,88 +256,119 @@ __device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_
 __device__ __forceinline__ float &el(float4 &v, int e) { return (&v.x)[e]; }
 __device__ __forceinline__ float elc(const float4 &v, int e) { return (&v.x)[e]; }
 
+// The group a lane of a vector kernel updates: either the flattened plane mapping above
+// (the whole volume, one z-plane per workgroup) or entry t of a list of groups (the CPML
+// shell; entry q is the group whose first cell is 4q).  in_range is false for lanes past
+// the end; they still load from valid addresses and take part in the shuffles.
+struct LaneGroup { int64_t i0, j, k; bool in_range; };
+__device__ __forceinline__ LaneGroup lane_group(const uint32_t *__restrict__ shell, int64_t nshell, int64_t nx,
+                                                int64_t ny, int64_t bpp, int64_t nbz, bool remap, int tyv) {
+    LaneGroup r;
+    if (shell) {
+        const int64_t t = (int64_t)blockIdx.x * (64 * tyv) + threadIdx.y * 64 + threadIdx.x;
+        r.in_range = t < nshell;
+        const int64_t q = r.in_range ? (int64_t)shell[t] : 0;
+        const int64_t G = nx / VX;
+        const int64_t rest = q / G;
+        r.i0 = (q - rest * G) * VX;
+        r.j = rest % ny;
+        r.k = rest / ny;
+    } else {
+        const PlaneGroup pg = plane_group(nx, ny, bpp, nbz, remap, tyv);
+        r.i0 = pg.i0;
+        r.j = pg.j;
+        r.k = pg.bz;
+        r.in_range = pg.in_plane;
+    }
+    return r;
+}
+
+// Whether the neighbouring lane (delta = -1: previous, +1: next) loaded the adjacent group
+// of the same x row; only then may its values stand in for a memory read.
+__device__ __forceinline__ bool lane_holds(int64_t o, int delta) {
+    const uint32_t lo = (uint32_t)o, hi = (uint32_t)((uint64_t)o >> 32);
+    const uint32_t nlo = delta < 0 ? __shfl_up(lo, 1) : __shfl_down(lo, 1);
+    const uint32_t nhi = delta < 0 ? __shfl_up(hi, 1) : __shfl_down(hi, 1);
+    const int lane = threadIdx.x;
+    const bool edge = delta < 0 ? lane == 0 : lane == 63;
+    return !edge && (int64_t)(((uint64_t)nhi << 32) | nlo) == o + delta * VX;
+}
+
 template <typename P, int TYV>
 __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
     const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
-    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt_mu0, int64_t nbx,
-    int64_t nby, int64_t nbz, int remap) {
-    const int lane = threadIdx.x;
-    const PlaneGroup pg = plane_group(nx, ny, nbx, nbz, remap != 0, TYV);
-    const int64_t i0 = pg.i0, j = pg.j;
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, int64_t nbz, int remap,
+    const uint32_t *__restrict__ shell, int64_t nshell) {
+    __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
+    const LaneGroup lg = lane_group(shell, nshell, nx, ny, nbx, nbz, remap != 0, TYV);
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
-    const bool act = pg.in_plane && j < ny - 1;
-    const int64_t k0 = pg.bz * kchunk;
-    const int64_t k1 = min(k0 + kchunk, nz - 1);
-    if (k0 >= k1) return;   // uniform across the workgroup
+    const bool act = lg.in_range && lg.j < ny - 1 && lg.k < nz - 1;
+    const int64_t k = act ? lg.k : 0;
     const int64_t sy = nx, sz = nx * ny;
     const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
     const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
     const float *zk = (const float *)cp.ik[2], *za = (const float *)cp.a[2], *zb = (const float *)cp.b[2];
     P **ps = (P **)cp.psi;
     const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
-    const int64_t ic = act ? i0 : 0;   // clamp addresses of idle lanes
-    const int64_t jc = act ? j : (int64_t)0;   // rows 0 and 1 exist (ny >= 2)
+    const int64_t ic = act ? lg.i0 : 0;   // clamp addresses of idle lanes
+    const int64_t jc = act ? lg.j : (int64_t)0;   // rows 0 and 1 exist (ny >= 2)
     int64_t jsy;
     const int ry = pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(idcx + ic), xk4 = ld4(xk + ic);
     const float idy_ = idcy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
     const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
-    int64_t o = ic + sy * jc + sz * k0;
-    float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o);
-    for (int64_t k = k0; k < k1; ++k, o += sz) {
-        const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
-        const float4 ez_c = ld4(Ez + o);
-        const float4 ez_y = ld4(Ez + o + sy), ex_y = ld4(Ex + o + sy);
-        float4 hx = ld4nt(Hx + o), hy = ld4nt(Hy + o), hz = ld4nt(Hz + o);
-        // x neighbours of element 3: element 0 of the next lane's group
-        float ez_nx = __shfl_down(ez_c.x, 1), ey_nx = __shfl_down(ey_c.x, 1);
-        if (lane == 63 && ic + VX < nx) {
-            ez_nx = Ez[o + VX];
-            ey_nx = Ey[o + VX];
-        }
-        const float idz_ = idcz[k], zk_ = zk[k];
-        int64_t jsz;
-        const int rz = pml_slab(k, nz, wz, &jsz);
-        const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+    const int64_t o = ic + sy * jc + sz * k;
+    const float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o);
+    const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
+    const float4 ez_c = ld4(Ez + o);
+    const float4 ez_y = ld4(Ez + o + sy), ex_y = ld4(Ex + o + sy);
+    float4 hx = ld4nt(Hx + o), hy = ld4nt(Hy + o), hz = ld4nt(Hz + o);
+    // x neighbours of element 3: element 0 of the next lane's group, or memory
+    float ez_nx = __shfl_down(ez_c.x, 1), ey_nx = __shfl_down(ey_c.x, 1);
+    if (!lane_holds(o, 1) && ic + VX < nx) {
+        ez_nx = Ez[o + VX];
+        ey_nx = Ey[o + VX];
+    }
+    const float idz_ = idcz[k], zk_ = zk[k];
+    int64_t jsz;
+    const int rz = pml_slab(k, nz, wz, &jsz);
+    const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
 #pragma unroll
-        for (int e = 0; e < VX; ++e) {
-            const int64_t i = ic + e;
-            if (!act || i >= nx - 1) continue;   // H at i = nx-1 keeps its value
-            const float ezc = elc(ez_c, e), eyc = elc(ey_c, e), exc = elc(ex_c, e);
-            const float ez_xp = e < VX - 1 ? elc(ez_c, e + 1) : ez_nx;
-            const float ey_xp = e < VX - 1 ? elc(ey_c, e + 1) : ey_nx;
-            const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
-            const float dEz_dy = (elc(ez_y, e) - ezc) * idy_;
-            const float dEy_dz = (elc(ey_n, e) - eyc) * idz_;
-            const float dEx_dz = (elc(ex_n, e) - exc) * idz_;
-            const float dEz_dx = (ez_xp - ezc) * idx_;
-            const float dEy_dx = (ey_xp - eyc) * idx_;
-            const float dEx_dy = (elc(ex_y, e) - exc) * idy_;
-            int64_t jsx = 0;
-            const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
-            const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-            const int64_t ox = jsx + wx * (jc + ny * k);
-            const int64_t oy = i + nx * (jsy + wy * k);
-            const int64_t oz = i + nx * (jc + ny * jsz);
-            const float pHxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
-            const float pHxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
-            el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz));
-            const float pHyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dEx_dz, cp.sc, cp.isc);
-            const float pHyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dEz_dx, cp.sc, cp.isc);
-            el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx));
-            const float pHzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dEy_dx, cp.sc, cp.isc);
-            const float pHzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dEx_dy, cp.sc, cp.isc);
-            el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy));
-        }
-        if (act) {
-            st4nt(Hx + o, hx);
-            st4nt(Hy + o, hy);
-            st4nt(Hz + o, hz);
-        }
-        ex_c = ex_n;
-        ey_c = ey_n;
+    for (int e = 0; e < VX; ++e) {
+        const int64_t i = ic + e;
+        if (!act || i >= nx - 1) continue;   // H at i = nx-1 keeps its value
+        const float ezc = elc(ez_c, e), eyc = elc(ey_c, e), exc = elc(ex_c, e);
+        const float ez_xp = e < VX - 1 ? elc(ez_c, e + 1) : ez_nx;
+        const float ey_xp = e < VX - 1 ? elc(ey_c, e + 1) : ey_nx;
+        const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+        const float dEz_dy = (elc(ez_y, e) - ezc) * idy_;
+        const float dEy_dz = (elc(ey_n, e) - eyc) * idz_;
+        const float dEx_dz = (elc(ex_n, e) - exc) * idz_;
+        const float dEz_dx = (ez_xp - ezc) * idx_;
+        const float dEy_dx = (ey_xp - eyc) * idx_;
+        const float dEx_dy = (elc(ex_y, e) - exc) * idy_;
+        int64_t jsx = 0;
+        const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
+        const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
+        const int64_t ox = jsx + wx * (jc + ny * k);
+        const int64_t oy = i + nx * (jsy + wy * k);
+        const int64_t oz = i + nx * (jc + ny * jsz);
+        const float pHxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
+        const float pHxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
+        el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz));
+        const float pHyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dEx_dz, cp.sc, cp.isc);
+        const float pHyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dEz_dx, cp.sc, cp.isc);
+        el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx));
+        const float pHzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dEy_dx, cp.sc, cp.isc);
+        const float pHzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dEx_dy, cp.sc, cp.isc);
+        el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy));
+    }
+    if (act) {
+        st4nt(Hx + o, hx);
+        st4nt(Hy + o, hy);
+        st4nt(Hz + o, hz);
     }
 }
 
@@ -350,114 +382,251 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
     const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
     const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
-    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, int64_t nby,
-    int64_t nbz, int remap) {
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt, int64_t nbx, int64_t nbz, int remap,
+    const uint32_t *__restrict__ shell, int64_t nshell) {
+    __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
     __shared__ float4 tab[MI ? 256 : 1];
     if (MI) {
         for (int t = threadIdx.y * 64 + threadIdx.x; t < nmat; t += 64 * TYV) tab[t] = mtab[t];
         __syncthreads();
     }
     const int lane = threadIdx.x;
-    const PlaneGroup pg = plane_group(nx, ny, nbx, nbz, remap != 0, TYV);
-    const int64_t i0 = pg.i0, j = pg.j;
-    const bool act = pg.in_plane && j >= 1;
-    const int64_t k0 = max(pg.bz * kchunk, (int64_t)1);
-    const int64_t k1 = min(pg.bz * kchunk + kchunk, nz);
-    if (k0 >= k1) return;   // uniform across the workgroup
+    (void)lane;
+    const LaneGroup lg = lane_group(shell, nshell, nx, ny, nbx, nbz, remap != 0, TYV);
+    const bool act = lg.in_range && lg.j >= 1 && lg.k >= 1;
+    const int64_t k = act ? lg.k : 1;
     const int64_t sy = nx, sz = nx * ny;
     const float *xk = (const float *)cp.ik[0], *xa = (const float *)cp.a[0], *xb = (const float *)cp.b[0];
     const float *yk = (const float *)cp.ik[1], *ya = (const float *)cp.a[1], *yb = (const float *)cp.b[1];
     const float *zk = (const float *)cp.ik[2], *za = (const float *)cp.a[2], *zb = (const float *)cp.b[2];
     P **ps = (P **)cp.psi;
     const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
-    const int64_t ic = act ? i0 : 0;
-    const int64_t jc = act ? j : (int64_t)1;   // rows 0 and 1 exist (ny >= 2)
+    const int64_t ic = act ? lg.i0 : 0;
+    const int64_t jc = act ? lg.j : (int64_t)1;   // rows 0 and 1 exist (ny >= 2)
     int64_t jsy;
     const int ry = pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(iddx + ic), xk4 = ld4(xk + ic);
     const float idy_ = iddy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
     const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
-    int64_t o = ic + sy * jc + sz * k0;
-    float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
-    for (int64_t k = k0; k < k1; ++k, o += sz) {
-        const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
-        const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
-        float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
-        float4 iex4, iey4, iez4;
-        if (MI) {
-            const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
-            const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
-            const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
-            iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
-            iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
-            iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
-        } else {
-            iex4 = ld4nt(iex + o);
-            iey4 = ld4nt(iey + o);
-            iez4 = ld4nt(iez + o);
-        }
-        // x neighbours of element 0: element 3 of the previous lane's group
-        float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
-        if (lane == 0 && ic > 0) {
-            hz_px = Hz[o - 1];
-            hy_px = Hy[o - 1];
-        }
-        const float idz_ = iddz[k], zk_ = zk[k];
-        int64_t jsz;
-        const int rz = pml_slab(k, nz, wz, &jsz);
-        const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+    const int64_t o = ic + sy * jc + sz * k;
+    const float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
+    const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
+    const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
+    float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
+    float4 iex4, iey4, iez4;
+    if (MI) {
+        const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
+        const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+        const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+        iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+        iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+        iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+    } else {
+        iex4 = ld4nt(iex + o);
+        iey4 = ld4nt(iey + o);
+        iez4 = ld4nt(iez + o);
+    }
+    // x neighbours of element 0: element 3 of the previous lane's group, or memory
+    float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
+    if (!lane_holds(o, -1) && ic > 0) {
+        hz_px = Hz[o - 1];
+        hy_px = Hy[o - 1];
+    }
+    const float idz_ = iddz[k], zk_ = zk[k];
+    int64_t jsz;
+    const int rz = pml_slab(k, nz, wz, &jsz);
+    const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
 #pragma unroll
-        for (int e = 0; e < VX; ++e) {
-            const int64_t i = ic + e;
-            if (!act || i < 1) continue;   // E, D at i = 0 keep their values
-            const float hxc = elc(hx_c, e), hyc = elc(hy_c, e), hzc = elc(hz_c, e);
-            const float hz_xm = e > 0 ? elc(hz_c, e - 1) : hz_px;
-            const float hy_xm = e > 0 ? elc(hy_c, e - 1) : hy_px;
-            const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
-            const float dHz_dy = (hzc - elc(hz_y, e)) * idy_;
-            const float dHy_dz = (hyc - elc(hy_p, e)) * idz_;
-            const float dHx_dz = (hxc - elc(hx_p, e)) * idz_;
-            const float dHz_dx = (hzc - hz_xm) * idx_;
-            const float dHy_dx = (hyc - hy_xm) * idx_;
-            const float dHx_dy = (hxc - elc(hx_y, e)) * idy_;
-            int64_t jsx = 0;
-            const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
-            const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-            const int64_t ox = jsx + wx * (jc + ny * k);
-            const int64_t oy = i + nx * (jsy + wy * k);
-            const int64_t oz = i + nx * (jc + ny * jsz);
-            const float pDxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
-            const float pDxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
-            const float cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
-            const float pDyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dHx_dz, cp.sc, cp.isc);
-            const float pDyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dHz_dx, cp.sc, cp.isc);
-            const float cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
-            const float pDzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dHy_dx, cp.sc, cp.isc);
-            const float pDzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dHx_dy, cp.sc, cp.isc);
-            const float cz_term = (dHy_dx * xk_ + pDzx) - (dHx_dy * yk_ + pDzy);
-            const float dxn = el(dx, e) + dt * cx_term;
-            const float dyn = el(dy, e) + dt * cy_term;
-            const float dzn = el(dz, e) + dt * cz_term;
-            el(dx, e) = dxn;
-            el(dy, e) = dyn;
-            el(dz, e) = dzn;
+    for (int e = 0; e < VX; ++e) {
+        const int64_t i = ic + e;
+        if (!act || i < 1) continue;   // E, D at i = 0 keep their values
+        const float hxc = elc(hx_c, e), hyc = elc(hy_c, e), hzc = elc(hz_c, e);
+        const float hz_xm = e > 0 ? elc(hz_c, e - 1) : hz_px;
+        const float hy_xm = e > 0 ? elc(hy_c, e - 1) : hy_px;
+        const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+        const float dHz_dy = (hzc - elc(hz_y, e)) * idy_;
+        const float dHy_dz = (hyc - elc(hy_p, e)) * idz_;
+        const float dHx_dz = (hxc - elc(hx_p, e)) * idz_;
+        const float dHz_dx = (hzc - hz_xm) * idx_;
+        const float dHy_dx = (hyc - hy_xm) * idx_;
+        const float dHx_dy = (hxc - elc(hx_y, e)) * idy_;
+        int64_t jsx = 0;
+        const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
+        const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
+        const int64_t ox = jsx + wx * (jc + ny * k);
+        const int64_t oy = i + nx * (jsy + wy * k);
+        const int64_t oz = i + nx * (jc + ny * jsz);
+        const float pDxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
+        const float pDxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
+        const float cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
+        const float pDyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dHx_dz, cp.sc, cp.isc);
+        const float pDyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dHz_dx, cp.sc, cp.isc);
+        const float cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
+        const float pDzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dHy_dx, cp.sc, cp.isc);
+        const float pDzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dHx_dy, cp.sc, cp.isc);
+        const float cz_term = (dHy_dx * xk_ + pDzx) - (dHx_dy * yk_ + pDzy);
+        const float dxn = el(dx, e) + dt * cx_term;
+        const float dyn = el(dy, e) + dt * cy_term;
+        const float dzn = el(dz, e) + dt * cz_term;
+        el(dx, e) = dxn;
+        el(dy, e) = dyn;
+        el(dz, e) = dzn;
+    }
+    // E = D * inv_eps, formed after the loop so E does not occupy registers through it
+    if (act && ic > 0) {
+        st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
+        st4nt(Ex + o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
+        st4nt(Ey + o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
+        st4nt(Ez + o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
+    } else if (act) {   // the group holding i = 0: that cell is not part of the E update
+        for (int e = 1; e < VX; ++e) {
+            Dx[o + e] = elc(dx, e); Dy[o + e] = elc(dy, e); Dz[o + e] = elc(dz, e);
+            Ex[o + e] = elc(dx, e) * elc(iex4, e);
+            Ey[o + e] = elc(dy, e) * elc(iey4, e);
+            Ez[o + e] = elc(dz, e) * elc(iez4, e);
         }
-        // E = D * inv_eps, formed after the loop so E does not occupy registers through it
-        if (act && ic > 0) {
-            st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
-            st4nt(Ex + o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
-            st4nt(Ey + o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
-            st4nt(Ez + o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
-        } else if (act) {   // the group holding i = 0: that cell is not part of the E update
-            for (int e = 1; e < VX; ++e) {
-                Dx[o + e] = elc(dx, e); Dy[o + e] = elc(dy, e); Dz[o + e] = elc(dz, e);
-                Ex[o + e] = elc(dx, e) * elc(iex4, e);
-                Ey[o + e] = elc(dy, e) * elc(iey4, e);
-                Ez[o + e] = elc(dz, e) * elc(iez4, e);
-            }
-        }
-        hx_p = hx_c;
-        hy_p = hy_c;
+    }
+}
+
+// ---- interior kernels ---------------------------------------------------------------------
+// The interior box holds the whole float4 groups that lie in no CPML slab and contain no
+// boundary cell on any axis.  There every psi term is +0 and no cell is skipped, so these
+// kernels run the reference's arithmetic with the psi terms added as +0 (bit for bit what
+// the general kernels compute there) without the slab tests, psi offsets and skip branches;
+// they need far fewer registers.  Lanes take consecutive groups of the box, row after row
+// and plane after plane, so a wave's loads are contiguous runs.  The CPML shell around the
+// box is updated by the general kernels above from a list of its groups.
+__device__ __forceinline__ void box_pos(uint32_t t, uint32_t per_plane, uint32_t ngi, uint32_t &kk, uint32_t &jj,
+                                        uint32_t &gg) {
+    kk = t / per_plane;
+    const uint32_t r = t - kk * per_plane;
+    jj = r / ngi;
+    gg = r - jj * ngi;
+}
+
+__global__ __launch_bounds__(256) void yee_H_int_kernel(
+    const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
+    float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
+    const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
+    const float *__restrict__ xk, const float *__restrict__ yk, const float *__restrict__ zk, int64_t nx,
+    int64_t ny, uint32_t gi0, uint32_t ngi, uint32_t j0, uint32_t per_plane, uint32_t k0, uint32_t ntot,
+    float dt_mu0) {
+    __builtin_amdgcn_s_setprio(3);
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const bool act = t < ntot;
+    uint32_t kk, jj, gg;
+    box_pos(act ? t : 0u, per_plane, ngi, kk, jj, gg);
+    const int64_t i0 = (int64_t)(gi0 + gg) * VX, j = (int64_t)(j0 + jj), k = (int64_t)(k0 + kk);
+    const int64_t sy = nx, sz = nx * ny;
+    const int64_t o = i0 + sy * j + sz * k;
+    const float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o), ez_c = ld4(Ez + o);
+    const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
+    const float4 ez_y = ld4(Ez + o + sy), ex_y = ld4(Ex + o + sy);
+    float4 hx = ld4nt(Hx + o), hy = ld4nt(Hy + o), hz = ld4nt(Hz + o);
+    // the next lane holds the next group of this row unless this is the row's last group
+    float ez_nx = __shfl_down(ez_c.x, 1), ey_nx = __shfl_down(ey_c.x, 1);
+    if ((threadIdx.x & 63) == 63 || gg + 1 == ngi) {
+        ez_nx = Ez[o + VX];
+        ey_nx = Ey[o + VX];
+    }
+    const float4 idx4 = ld4(idcx + i0), xk4 = ld4(xk + i0);
+    const float idy_ = idcy[j], yk_ = yk[j], idz_ = idcz[k], zk_ = zk[k];
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+        const float ezc = elc(ez_c, e), eyc = elc(ey_c, e), exc = elc(ex_c, e);
+        const float ez_xp = e < VX - 1 ? elc(ez_c, e + 1) : ez_nx;
+        const float ey_xp = e < VX - 1 ? elc(ey_c, e + 1) : ey_nx;
+        const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+        const float dEz_dy = (elc(ez_y, e) - ezc) * idy_;
+        const float dEy_dz = (elc(ey_n, e) - eyc) * idz_;
+        const float dEx_dz = (elc(ex_n, e) - exc) * idz_;
+        const float dEz_dx = (ez_xp - ezc) * idx_;
+        const float dEy_dx = (ey_xp - eyc) * idx_;
+        const float dEx_dy = (elc(ex_y, e) - exc) * idy_;
+        el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + 0.0f) - (dEy_dz * zk_ + 0.0f));
+        el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + 0.0f) - (dEz_dx * xk_ + 0.0f));
+        el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + 0.0f) - (dEx_dy * yk_ + 0.0f));
+    }
+    if (act) {
+        st4nt(Hx + o, hx);
+        st4nt(Hy + o, hy);
+        st4nt(Hz + o, hz);
+    }
+}
+
+template <bool MI>
+__global__ __launch_bounds__(256) void yee_E_int_kernel(
+    float *__restrict__ Ex, float *__restrict__ Ey, float *__restrict__ Ez,
+    const float *__restrict__ Hx, const float *__restrict__ Hy, const float *__restrict__ Hz,
+    float *__restrict__ Dx, float *__restrict__ Dy, float *__restrict__ Dz,
+    const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
+    const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
+    const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
+    const float *__restrict__ xk, const float *__restrict__ yk, const float *__restrict__ zk, int64_t nx,
+    int64_t ny, uint32_t gi0, uint32_t ngi, uint32_t j0, uint32_t per_plane, uint32_t k0, uint32_t ntot,
+    float dt) {
+    __builtin_amdgcn_s_setprio(3);
+    __shared__ float4 tab[MI ? 256 : 1];
+    if (MI) {
+        for (int t = threadIdx.x; t < nmat; t += 256) tab[t] = mtab[t];
+        __syncthreads();
+    }
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const bool act = t < ntot;
+    uint32_t kk, jj, gg;
+    box_pos(act ? t : 0u, per_plane, ngi, kk, jj, gg);
+    const int64_t i0 = (int64_t)(gi0 + gg) * VX, j = (int64_t)(j0 + jj), k = (int64_t)(k0 + kk);
+    const int64_t sy = nx, sz = nx * ny;
+    const int64_t o = i0 + sy * j + sz * k;
+    const float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
+    const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
+    const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
+    float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
+    float4 iex4, iey4, iez4;
+    if (MI) {
+        const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
+        const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+        const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+        iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+        iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+        iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+    } else {
+        iex4 = ld4nt(iex + o);
+        iey4 = ld4nt(iey + o);
+        iez4 = ld4nt(iez + o);
+    }
+    // the previous lane holds the previous group of this row unless this is its first group
+    float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
+    if ((threadIdx.x & 63) == 0 || gg == 0) {
+        hz_px = Hz[o - 1];
+        hy_px = Hy[o - 1];
+    }
+    const float4 idx4 = ld4(iddx + i0), xk4 = ld4(xk + i0);
+    const float idy_ = iddy[j], yk_ = yk[j], idz_ = iddz[k], zk_ = zk[k];
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+        const float hxc = elc(hx_c, e), hyc = elc(hy_c, e), hzc = elc(hz_c, e);
+        const float hz_xm = e > 0 ? elc(hz_c, e - 1) : hz_px;
+        const float hy_xm = e > 0 ? elc(hy_c, e - 1) : hy_px;
+        const float idx_ = elc(idx4, e), xk_ = elc(xk4, e);
+        const float dHz_dy = (hzc - elc(hz_y, e)) * idy_;
+        const float dHy_dz = (hyc - elc(hy_p, e)) * idz_;
+        const float dHx_dz = (hxc - elc(hx_p, e)) * idz_;
+        const float dHz_dx = (hzc - hz_xm) * idx_;
+        const float dHy_dx = (hyc - hy_xm) * idx_;
+        const float dHx_dy = (hxc - elc(hx_y, e)) * idy_;
+        const float cx_term = (dHz_dy * yk_ + 0.0f) - (dHy_dz * zk_ + 0.0f);
+        const float cy_term = (dHx_dz * zk_ + 0.0f) - (dHz_dx * xk_ + 0.0f);
+        const float cz_term = (dHy_dx * xk_ + 0.0f) - (dHx_dy * yk_ + 0.0f);
+        el(dx, e) = el(dx, e) + dt * cx_term;
+        el(dy, e) = el(dy, e) + dt * cy_term;
+        el(dz, e) = el(dz, e) + dt * cz_term;
+    }
+    if (act) {
+        st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
+        st4nt(Ex + o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
+        st4nt(Ey + o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
+        st4nt(Ez + o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
     }
 }
 
@@ -500,17 +669,6 @@ static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
     return MP_OK;
 }
 
-// Vector-form launch shape: z chunks of MP_YEE_KCHUNK planes per workgroup (default 1: on
-// the 512 x 512 x 256 step, chunks of 1 / 2 / 4 / 8 planes give an E update of
-// 793 / 817 / 840 / 880 us).
-static int64_t vec_kchunk() {
-    static int64_t kc = [] {
-        const char *e = getenv("MP_YEE_KCHUNK");
-        const long v = e ? atol(e) : 1;
-        return (int64_t)(v > 0 ? v : 1);
-    }();
-    return kc;
-}
 // Measured on the 512x512x256 step the remap costs ~3% (the two x tiles of a row then sit
 // on different XCDs), so it is opt-in (MP_YEE_REMAP=1).
 static int xcd_remap() {
@@ -528,22 +686,119 @@ static int yee_tyv() {
     return t;
 }
 
+// Interior box (groups gi0 .. gi0+ngi-1 of rows j0 .. j0+nj-1 of planes k0 .. k0+nk-1): the
+// float4 groups outside every CPML slab and off every boundary cell (a margin of at least one
+// cell on each face, so i, j, k >= 1 and <= n-2 throughout).
+struct Box {
+    int64_t gi0, ngi, j0, nj, k0, nk;
+    int64_t n() const { return ngi * nj * nk; }
+};
+static Box interior_box(const mp_yee *g, const mp_cpml *c) {
+    const int64_t mx = std::max<int64_t>(c->npml_x, 1), my = std::max<int64_t>(c->npml_y, 1);
+    const int64_t mz = std::max<int64_t>(c->npml_z, 1);
+    Box b;
+    b.gi0 = (mx + VX - 1) / VX;
+    b.ngi = std::max<int64_t>((g->nx - mx) / VX - b.gi0, 0);
+    b.j0 = my;
+    b.nj = std::max<int64_t>(g->ny - 2 * my, 0);
+    b.k0 = mz;
+    b.nk = std::max<int64_t>(g->nz - 2 * mz, 0);
+    return b;
+}
+
+// The split into interior and shell launches is used when the box is non-empty and every
+// group index fits 32 bits (MP_YEE_NO_SPLIT=1 keeps one launch over the whole volume).
+static bool split_ok(const mp_yee *g, const Box &b) {
+    static const bool off = getenv("MP_YEE_NO_SPLIT") != nullptr;
+    return !off && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32);
+}
+
+// Device list of the shell groups (every group of the volume outside the box, in memory
+// order), built on the host once per device and grid shape and kept until the process ends.
+struct ShellList {
+    int dev;
+    int64_t nx, ny, nz, gi0, ngi, j0, nj, k0, nk;
+    uint32_t *d;
+    int64_t n;
+};
+static std::vector<ShellList> g_shells;
+static std::mutex g_shell_mu;
+
+static int shell_list(const mp_yee *g, const Box &b, const uint32_t **out, int64_t *nout) {
+    int dev = 0;
+    MP_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_shell_mu);
+    for (const ShellList &s : g_shells) {
+        if (s.dev == dev && s.nx == g->nx && s.ny == g->ny && s.nz == g->nz && s.gi0 == b.gi0 && s.ngi == b.ngi &&
+            s.j0 == b.j0 && s.nj == b.nj && s.k0 == b.k0 && s.nk == b.nk) {
+            *out = s.d;
+            *nout = s.n;
+            return MP_OK;
+        }
+    }
+    const int64_t G = g->nx / VX, ny = g->ny, nz = g->nz;
+    std::vector<uint32_t> h;
+    h.reserve((size_t)(G * ny * nz - b.n()));
+    for (int64_t k = 0; k < nz; ++k) {
+        const bool kin = k >= b.k0 && k < b.k0 + b.nk;
+        for (int64_t j = 0; j < ny; ++j) {
+            const int64_t base = G * (j + ny * k);
+            if (kin && j >= b.j0 && j < b.j0 + b.nj) {
+                for (int64_t q = 0; q < b.gi0; ++q) h.push_back((uint32_t)(base + q));
+                for (int64_t q = b.gi0 + b.ngi; q < G; ++q) h.push_back((uint32_t)(base + q));
+            } else {
+                for (int64_t q = 0; q < G; ++q) h.push_back((uint32_t)(base + q));
+            }
+        }
+    }
+    ShellList s{dev, g->nx, g->ny, g->nz, b.gi0, b.ngi, b.j0, b.nj, b.k0, b.nk, nullptr, (int64_t)h.size()};
+    if (s.n > 0) {
+        MP_HIP(hipMalloc((void **)&s.d, h.size() * sizeof(uint32_t)));
+        MP_HIP(hipMemcpy(s.d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    g_shells.push_back(s);
+    *out = s.d;
+    *nout = s.n;
+    return MP_OK;
+}
+
 template <typename P, int TY>
 static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:60-101)
-    const int64_t kc = vec_kchunk();
-    // nbx = workgroups per z-plane of the flattened group mapping; nby is unused (1)
-    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nby = 1, nbz = cdiv(nz, kc);
-    dim3 block(64, TY, 1);
     const float dt_mu0 = (float)dt / (float)mu0;
+    const Box b = interior_box(g, c);
+    dim3 block(64, TY, 1);
+    if (!split_ok(g, b)) {
+        // one launch: one z-plane per workgroup range (nbx workgroups per plane)
+        const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
+        KernelTimer t("yee_H");
+        yee_H_vec_kernel<P, TY><<<(unsigned)(nbx * nbz), block, 0, stream()>>>(
+            (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+            (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, nbz, xcd_remap(),
+            nullptr, 0);
+        MP_LAUNCH_CHECK("yee_H_vec_kernel");
+        return MP_OK;
+    }
+    const uint32_t *shell = nullptr;
+    int64_t nshell = 0;
+    if (int e = shell_list(g, b, &shell, &nshell)) return e;
     KernelTimer t("yee_H");
-    yee_H_vec_kernel<P, TY><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(
+    yee_H_int_kernel<<<cdiv(b.n(), 256), 256, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
         (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, kc, dt_mu0, nbx, nby, nbz,
-        xcd_remap());
-    MP_LAUNCH_CHECK("yee_H_vec_kernel");
+        (const float *)g->inv_d_cell_z, (const float *)c->inv_kappa[0], (const float *)c->inv_kappa[1],
+        (const float *)c->inv_kappa[2], nx, ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0,
+        (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0, (uint32_t)b.n(), dt_mu0);
+    MP_LAUNCH_CHECK("yee_H_int_kernel");
+    if (nshell > 0) {
+        yee_H_vec_kernel<P, TY><<<cdiv(nshell, 64 * TY), block, 0, stream()>>>(
+            (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+            (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, 1, 0, shell, nshell);
+        MP_LAUNCH_CHECK("yee_H_vec_kernel");
+    }
     return MP_OK;
 }
 
@@ -551,23 +806,52 @@ template <typename P, int TY>
 static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:103-152)
-    const int64_t kc = vec_kchunk();
-    const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nby = 1, nbz = cdiv(nz, kc);
-    dim3 block(64, TY, 1);
-    KernelTimer t("yee_E");
     const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
                     getenv("MP_YEE_NO_EPS_INDEX") == nullptr;
+    const Box b = interior_box(g, c);
+    dim3 block(64, TY, 1);
 #define E_ARGS                                                                                             \
     (float *)g->ex, (float *)g->ey, (float *)g->ez, (const float *)g->hx, (const float *)g->hy,            \
         (const float *)g->hz, (float *)g->dx, (float *)g->dy, (float *)g->dz, (const float *)g->inv_eps_x, \
         (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, g->eps_index,                            \
         (const float4 *)g->eps_table, g->n_eps, (const float *)g->inv_d_dual_x,                            \
-        (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz, \
-        kc, (float)dt, nbx, nby, nbz, xcd_remap()
-    if (mi) yee_E_vec_kernel<P, TY, true><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
-    else yee_E_vec_kernel<P, TY, false><<<(unsigned)(nbx * nby * nbz), block, 0, stream()>>>(E_ARGS);
+        (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z
+    if (!split_ok(g, b)) {
+        const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
+        const unsigned grid = (unsigned)(nbx * nbz);
+        KernelTimer t("yee_E");
+        if (mi)
+            yee_E_vec_kernel<P, TY, true><<<grid, block, 0, stream()>>>(
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, (float)dt, nbx, nbz, xcd_remap(), nullptr, 0);
+        else
+            yee_E_vec_kernel<P, TY, false><<<grid, block, 0, stream()>>>(
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, (float)dt, nbx, nbz, xcd_remap(), nullptr, 0);
+        MP_LAUNCH_CHECK("yee_E_vec_kernel");
+        return MP_OK;
+    }
+    const uint32_t *shell = nullptr;
+    int64_t nshell = 0;
+    if (int e = shell_list(g, b, &shell, &nshell)) return e;
+    KernelTimer t("yee_E");
+#define INT_ARGS                                                                                          \
+    (const float *)c->inv_kappa[0], (const float *)c->inv_kappa[1], (const float *)c->inv_kappa[2], nx, ny, \
+        (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0, (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0,         \
+        (uint32_t)b.n(), (float)dt
+    if (mi) yee_E_int_kernel<true><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
+    else yee_E_int_kernel<false><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
+    MP_LAUNCH_CHECK("yee_E_int_kernel");
+    if (nshell > 0) {
+        const unsigned grid = cdiv(nshell, 64 * TY);
+        if (mi)
+            yee_E_vec_kernel<P, TY, true><<<grid, block, 0, stream()>>>(
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, (float)dt, 1, 1, 0, shell, nshell);
+        else
+            yee_E_vec_kernel<P, TY, false><<<grid, block, 0, stream()>>>(
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, (float)dt, 1, 1, 0, shell, nshell);
+        MP_LAUNCH_CHECK("yee_E_vec_kernel");
+    }
+#undef INT_ARGS
 #undef E_ARGS
-    MP_LAUNCH_CHECK("yee_E_vec_kernel");
     return MP_OK;
 }
 
