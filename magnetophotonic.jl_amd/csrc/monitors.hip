@@ -81,10 +81,21 @@ constexpr int DFT_MAXNB_STEPS = 64;  // longest batch
 
 template <typename S> struct Quad { S a, b, c, d; };
 
+// The step's snapshot of every plane's tangential fields; block (0, 0) also writes the
+// step's phase slot, the nb factors (cos wt * dt, -sin wt * dt).
 template <typename S>
 __global__ __launch_bounds__(256) void dft_snapshot_kernel(PlaneSet ps, const S *Ex, const S *Ey, const S *Ez,
                                                            const S *Hx, const S *Hy, const S *Hz,
-                                                           Quad<S> *__restrict__ snap, int64_t nx, int64_t ny) {
+                                                           Quad<S> *__restrict__ snap, int64_t nx, int64_t ny,
+                                                           double *__restrict__ phase,
+                                                           const double *__restrict__ omega, int nb, double t,
+                                                           double dt) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && (int)threadIdx.x < nb) {
+        const int q = threadIdx.x;
+        const double ph = omega[q] * t;
+        phase[2 * q] = cos(ph) * dt;
+        phase[2 * q + 1] = -sin(ph) * dt;
+    }
     const PlaneDesc &pl = ps.p[blockIdx.y];
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t np = pl.n1 * pl.n2;
@@ -98,14 +109,6 @@ __global__ __launch_bounds__(256) void dft_snapshot_kernel(PlaneSet ps, const S 
     snap[pl.off + c] = Quad<S>{Ea[o], Eb[o], Ha[o], Hb[o]};
 }
 
-// One phase slot: the nb factors (cos wt * dt, -sin wt * dt) of one step.
-__global__ void phase_slot_kernel(double *phase, const double *omega, int nb, double t, double dt) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nb) return;
-    const double ph = omega[q] * t;
-    phase[2 * q] = cos(ph) * dt;
-    phase[2 * q + 1] = -sin(ph) * dt;
-}
 
 template <typename S, int NB>
 __global__ __launch_bounds__(256) void dft_batch_kernel(PlaneSet ps, const Quad<S> *__restrict__ snap,
@@ -584,23 +587,19 @@ int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
     if (K > DFT_MAXNB_STEPS) return fail_arg("dft record: batch longer than %d steps", DFT_MAXNB_STEPS);
     if (!g_dft.phase) MP_HIP(hipMalloc((void **)&g_dft.phase, sizeof(double) * 2 * DFT_MAXNB * DFT_MAXNB_STEPS));
     {
-        KernelTimer t0("dft_phase");
-        phase_slot_kernel<<<1, 64, 0, stream()>>>(g_dft.phase + 2 * DFT_MAXNB * g_dft.count, omega_bins, nb, t, dt);
-        MP_LAUNCH_CHECK("phase_slot_kernel");
-    }
-    {
         dim3 grid(cdiv(maxnp, 256), nplanes);
+        double *slot = g_dft.phase + 2 * DFT_MAXNB * g_dft.count;
         KernelTimer t1("dft_snapshot");
         if (g->field_dtype == MP_F32)
             dft_snapshot_kernel<float><<<grid, 256, 0, stream()>>>(
                 ps, (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (const float *)g->hx,
                 (const float *)g->hy, (const float *)g->hz, (Quad<float> *)g_dft.snap + total * g_dft.count, g->nx,
-                g->ny);
+                g->ny, slot, omega_bins, nb, t, dt);
         else
             dft_snapshot_kernel<double><<<grid, 256, 0, stream()>>>(
                 ps, (const double *)g->ex, (const double *)g->ey, (const double *)g->ez, (const double *)g->hx,
                 (const double *)g->hy, (const double *)g->hz, (Quad<double> *)g_dft.snap + total * g_dft.count,
-                g->nx, g->ny);
+                g->nx, g->ny, slot, omega_bins, nb, t, dt);
         MP_LAUNCH_CHECK("dft_snapshot_kernel");
     }
     if (++g_dft.count == K) return dft_flush(g, planes, nplanes, nb, dt);

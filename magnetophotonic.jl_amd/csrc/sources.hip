@@ -33,9 +33,13 @@ __global__ void inject_mode_x_kernel(S *E, S *D, const S *ie, S *H, const C *pro
 
 // Plane injection: E[c] = S(double(E[c]) + dv), D[c] = S(double(D[c]) + dv / double(ie[c]))
 // over every cell of plane `idx` of `axis` (Source.jl:216-236 promotes to Float64).
+// blockIdx.z selects one of the two interpolation planes (distinct planes: no cell is
+// touched twice within a launch).
 template <typename S>
-__global__ void inject_plane_kernel(S *F, S *D, const S *ie, int axis, int64_t idx, int64_t nx,
-                                    int64_t ny, int64_t nz, double dv) {
+__global__ void inject_plane_kernel(S *F, S *D, const S *ie, int axis, int64_t idx0, double dv0, int64_t idx1,
+                                    double dv1, int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t idx = blockIdx.z ? idx1 : idx0;
+    const double dv = blockIdx.z ? dv1 : dv0;
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t b = (int64_t)blockIdx.y * blockDim.y + threadIdx.y;
     int64_t o;
@@ -144,23 +148,24 @@ MP_API int mp_inject_plane_3d(const mp_yee *g, int32_t component, int32_t axis, 
     void *F = p.E ? p.E : p.H;
     const int64_t n1 = axis == 0 ? g->ny : g->nx;
     const int64_t n2 = axis == 2 ? g->ny : g->nz;
+    if (g->field_dtype != MP_F32 && g->field_dtype != MP_F64)
+        return fail_arg("mp_inject_plane_3d: unsupported field dtype");
     dim3 block(64, 4);
-    dim3 grid(cdiv(n1, 64), cdiv(n2, 4));
-    const int64_t idxs[2] = {i0, i1};
-    const double ws[2] = {w0, w1};
+    const double dv0 = w0 * value, dv1 = w1 * value;
+    // both planes in one launch when they differ; the same plane twice keeps the order
+    const bool both = i1 >= 0 && i1 != i0;
+    const int passes = (i1 >= 0 && !both) ? 2 : 1;
     KernelTimer t("inject_plane");
-    for (int s = 0; s < 2; ++s) {
-        if (idxs[s] < 0) continue;
-        const double dv = ws[s] * value;
-        if (g->field_dtype == MP_F32) {
-            inject_plane_kernel<float><<<grid, block, 0, stream()>>>((float *)F, (float *)p.D,
-                (const float *)p.ie, axis, idxs[s], g->nx, g->ny, g->nz, dv);
-        } else if (g->field_dtype == MP_F64) {
+    for (int s = 0; s < passes; ++s) {
+        dim3 grid(cdiv(n1, 64), cdiv(n2, 4), both ? 2 : 1);
+        const double d0 = s == 0 ? dv0 : dv1;
+        if (g->field_dtype == MP_F32)
+            inject_plane_kernel<float><<<grid, block, 0, stream()>>>((float *)F, (float *)p.D, (const float *)p.ie,
+                                                                     axis, i0, d0, i1, dv1, g->nx, g->ny, g->nz);
+        else
             inject_plane_kernel<double><<<grid, block, 0, stream()>>>((double *)F, (double *)p.D,
-                (const double *)p.ie, axis, idxs[s], g->nx, g->ny, g->nz, dv);
-        } else {
-            return fail_arg("mp_inject_plane_3d: unsupported field dtype");
-        }
+                                                                      (const double *)p.ie, axis, i0, d0, i1, dv1,
+                                                                      g->nx, g->ny, g->nz);
         MP_LAUNCH_CHECK("inject_plane_kernel");
     }
     return MP_OK;
