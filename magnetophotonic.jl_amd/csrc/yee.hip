@@ -265,14 +265,17 @@ __device__ __forceinline__ LaneGroup lane_group(const uint32_t *__restrict__ she
                                                 int64_t ny, int64_t bpp, int64_t nbz, bool remap, int tyv) {
     LaneGroup r;
     if (shell) {
+        // list entries, rows and planes fit 32 bits (the launcher checks), so the index
+        // arithmetic stays in 32-bit integers
         const int64_t t = (int64_t)blockIdx.x * (64 * tyv) + threadIdx.y * 64 + threadIdx.x;
         r.in_range = t < nshell;
-        const int64_t q = r.in_range ? (int64_t)shell[t] : 0;
-        const int64_t G = nx / VX;
-        const int64_t rest = q / G;
-        r.i0 = (q - rest * G) * VX;
-        r.j = rest % ny;
-        r.k = rest / ny;
+        const uint32_t q = r.in_range ? shell[t] : 0u;
+        const uint32_t G = (uint32_t)(nx / VX), ny32 = (uint32_t)ny;
+        const uint32_t rest = q / G;
+        const uint32_t kq = rest / ny32;
+        r.i0 = (int64_t)(q - rest * G) * VX;
+        r.j = (int64_t)(rest - kq * ny32);
+        r.k = (int64_t)kq;
     } else {
         const PlaneGroup pg = plane_group(nx, ny, bpp, nbz, remap, tyv);
         r.i0 = pg.i0;
