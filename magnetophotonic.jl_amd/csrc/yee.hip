@@ -256,6 +256,74 @@ __device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_
 __device__ __forceinline__ float &el(float4 &v, int e) { return (&v.x)[e]; }
 __device__ __forceinline__ float elc(const float4 &v, int e) { return (&v.x)[e]; }
 
+// psi of the four cells of a group in a y or z slab.  The slab coordinate of those slabs
+// depends on j or k only, so the four values sit side by side (at i0 + nx * (...), a
+// multiple of four) and move as one vector access instead of four strided ones.  A cell
+// that the update skips keeps its stored bits: they are written back unchanged.
+template <typename P> struct alignas(4 * sizeof(P)) Psi4 { P v[4]; };
+template <typename P> __device__ __forceinline__ Psi4<P> psi4_load(const P *p, int64_t o) {
+    return *reinterpret_cast<const Psi4<P> *>(p + o);
+}
+template <typename P> __device__ __forceinline__ void psi4_store(P *p, int64_t o, const Psi4<P> &q) {
+    *reinterpret_cast<Psi4<P> *>(p + o) = q;
+}
+__device__ __forceinline__ float psi_val(float x, float) { return x; }
+__device__ __forceinline__ float psi_val(bf16 x, float isc) { return ld<float>(x) * isc; }
+__device__ __forceinline__ float psi_val(f16s x, float isc) { return ld<float>(x) * isc; }
+__device__ __forceinline__ void psi_put(float &x, float v, float) { x = v; }
+__device__ __forceinline__ void psi_put(bf16 &x, float v, float sc) { x = st<bf16>(v * sc); }
+__device__ __forceinline__ void psi_put(f16s &x, float v, float sc) { x = st<f16s>(v * sc); }
+// _cpml_term_*! (Kernels.jl:16-56) on element e of a loaded quad: psi = fma(b, psi, a*deriv)
+template <typename P>
+__device__ __forceinline__ float psi4_step(Psi4<P> &q, int e, float b, float a, float deriv, float sc, float isc) {
+    const float p = fmaT<float>(b, psi_val(q.v[e], isc), a * deriv);
+    psi_put(q.v[e], p, sc);
+    return p;
+}
+
+// x-slab psi of the group's cells (arrays 6/7: the y component's x term, 8/9: the z
+// component's), loaded before the update so that no load waits inside a per-cell branch.  A
+// cell outside the slab reads the row's first slab entry (a valid address) and stores
+// nothing.
+template <typename P> struct PsiX {
+    int r[VX];
+    int64_t off[VX];
+    P q6[VX], q8[VX];
+    float a[VX], b[VX];
+};
+template <typename P>
+__device__ __forceinline__ void psix_load(PsiX<P> &x, P *const *ps, bool load, bool xslab, int64_t ic, int64_t nx,
+                                          int64_t wx, int64_t row, const float *xa, const float *xb) {
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+        int64_t jsx = 0;
+        x.r[e] = xslab ? pml_slab(ic + e, nx, wx, &jsx) : 0;
+        x.off[e] = jsx + wx * row;
+        x.q6[e] = P{};
+        x.q8[e] = P{};
+        x.a[e] = 0.f;
+        x.b[e] = 0.f;
+    }
+    if (load && xslab) {
+#pragma unroll
+        for (int e = 0; e < VX; ++e) {
+            x.q6[e] = ps[x.r[e] == 2 ? 7 : 6][x.off[e]];
+            x.q8[e] = ps[x.r[e] == 2 ? 9 : 8][x.off[e]];
+            x.a[e] = xa[ic + e];
+            x.b[e] = xb[ic + e];
+        }
+    }
+}
+// _cpml_term_x! (Kernels.jl:16-56) on cell e: psi = fma(b, psi, a*deriv), stored at once
+template <typename P>
+__device__ __forceinline__ float psix_step(P &q, P *arr, int64_t off, float b, float a, float deriv, float sc,
+                                           float isc) {
+    const float p = fmaT<float>(b, psi_val(q, isc), a * deriv);
+    psi_put(q, p, sc);
+    arr[off] = q;
+    return p;
+}
+
 // The group a lane of a vector kernel updates: either the flattened plane mapping above
 // (the whole volume, one z-plane per workgroup) or entry t of a list of groups (the CPML
 // shell; entry q is the group whose first cell is 4q).  in_range is false for lanes past
@@ -338,6 +406,21 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     int64_t jsz;
     const int rz = pml_slab(k, nz, wz, &jsz);
     const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+    // y-slab psi of Hx and Hz, z-slab psi of Hx and Hy (lo or hi slab array)
+    const int64_t oy0 = ic + nx * (jsy + wy * k), oz0 = ic + nx * (jc + ny * jsz);
+    P *const pxy = ps[ry == 2 ? 1 : 0], *const pzy = ps[ry == 2 ? 11 : 10];
+    P *const pxz = ps[rz == 2 ? 3 : 2], *const pyz = ps[rz == 2 ? 5 : 4];
+    Psi4<P> qxy{}, qzy{}, qxz{}, qyz{};
+    if (act && ry) {
+        qxy = psi4_load(pxy, oy0);
+        qzy = psi4_load(pzy, oy0);
+    }
+    if (act && rz) {
+        qxz = psi4_load(pxz, oz0);
+        qyz = psi4_load(pyz, oz0);
+    }
+    PsiX<P> px;
+    psix_load(px, ps, act, xslab, ic, nx, wx, jc + ny * k, xa, xb);
 #pragma unroll
     for (int e = 0; e < VX; ++e) {
         const int64_t i = ic + e;
@@ -352,26 +435,32 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
         const float dEz_dx = (ez_xp - ezc) * idx_;
         const float dEy_dx = (ey_xp - eyc) * idx_;
         const float dEx_dy = (elc(ex_y, e) - exc) * idy_;
-        int64_t jsx = 0;
-        const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
-        const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-        const int64_t ox = jsx + wx * (jc + ny * k);
-        const int64_t oy = i + nx * (jsy + wy * k);
-        const int64_t oz = i + nx * (jc + ny * jsz);
-        const float pHxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dEz_dy, cp.sc, cp.isc);
-        const float pHxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dEy_dz, cp.sc, cp.isc);
+        const int rx = px.r[e];
+        const float pHxy = ry ? psi4_step(qxy, e, yb_, ya_, dEz_dy, cp.sc, cp.isc) : 0.f;
+        const float pHxz = rz ? psi4_step(qxz, e, zb_, za_, dEy_dz, cp.sc, cp.isc) : 0.f;
         el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + pHxy) - (dEy_dz * zk_ + pHxz));
-        const float pHyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dEx_dz, cp.sc, cp.isc);
-        const float pHyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dEz_dx, cp.sc, cp.isc);
+        const float pHyz = rz ? psi4_step(qyz, e, zb_, za_, dEx_dz, cp.sc, cp.isc) : 0.f;
+        float pHyx = 0.f, pHzx = 0.f;
+        if (rx) {
+            pHyx = psix_step(px.q6[e], ps[rx == 2 ? 7 : 6], px.off[e], px.b[e], px.a[e], dEz_dx, cp.sc, cp.isc);
+            pHzx = psix_step(px.q8[e], ps[rx == 2 ? 9 : 8], px.off[e], px.b[e], px.a[e], dEy_dx, cp.sc, cp.isc);
+        }
         el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx));
-        const float pHzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dEy_dx, cp.sc, cp.isc);
-        const float pHzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dEx_dy, cp.sc, cp.isc);
+        const float pHzy = ry ? psi4_step(qzy, e, yb_, ya_, dEx_dy, cp.sc, cp.isc) : 0.f;
         el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy));
     }
     if (act) {
         st4nt(Hx + o, hx);
         st4nt(Hy + o, hy);
         st4nt(Hz + o, hz);
+        if (ry) {
+            psi4_store(pxy, oy0, qxy);
+            psi4_store(pzy, oy0, qzy);
+        }
+        if (rz) {
+            psi4_store(pxz, oz0, qxz);
+            psi4_store(pyz, oz0, qyz);
+        }
     }
 }
 
@@ -455,6 +544,21 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
         int64_t jsz;
         const int rz = pml_slab(k, nz, wz, &jsz);
         const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
+        // y-slab psi of Dx and Dz, z-slab psi of Dx and Dy (lo or hi slab array)
+        const int64_t oy0 = ic + nx * (jsy + wy * k), oz0 = ic + nx * (jc + ny * jsz);
+        P *const pxy = ps[ry == 2 ? 1 : 0], *const pzy = ps[ry == 2 ? 11 : 10];
+        P *const pxz = ps[rz == 2 ? 3 : 2], *const pyz = ps[rz == 2 ? 5 : 4];
+        Psi4<P> qxy{}, qzy{}, qxz{}, qyz{};
+        if (act && ry) {
+            qxy = psi4_load(pxy, oy0);
+            qzy = psi4_load(pzy, oy0);
+        }
+        if (act && rz) {
+            qxz = psi4_load(pxz, oz0);
+            qyz = psi4_load(pyz, oz0);
+        }
+        PsiX<P> px;
+        psix_load(px, ps, act, xslab, ic, nx, wx, jc + ny * k, xa, xb);
 #pragma unroll
         for (int e = 0; e < VX; ++e) {
             const int64_t i = ic + e;
@@ -469,20 +573,18 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
             const float dHz_dx = (hzc - hz_xm) * idx_;
             const float dHy_dx = (hyc - hy_xm) * idx_;
             const float dHx_dy = (hxc - elc(hx_y, e)) * idy_;
-            int64_t jsx = 0;
-            const int rx = xslab ? pml_slab(i, nx, wx, &jsx) : 0;
-            const float xa_ = rx ? xa[i] : 0.f, xb_ = rx ? xb[i] : 0.f;
-            const int64_t ox = jsx + wx * (jc + ny * k);
-            const int64_t oy = i + nx * (jsy + wy * k);
-            const int64_t oz = i + nx * (jc + ny * jsz);
-            const float pDxy = cpml_term<float, P>(ps[0], ps[1], ry, oy, yb_, ya_, dHz_dy, cp.sc, cp.isc);
-            const float pDxz = cpml_term<float, P>(ps[2], ps[3], rz, oz, zb_, za_, dHy_dz, cp.sc, cp.isc);
+            const int rx = px.r[e];
+            const float pDxy = ry ? psi4_step(qxy, e, yb_, ya_, dHz_dy, cp.sc, cp.isc) : 0.f;
+            const float pDxz = rz ? psi4_step(qxz, e, zb_, za_, dHy_dz, cp.sc, cp.isc) : 0.f;
             const float cx_term = (dHz_dy * yk_ + pDxy) - (dHy_dz * zk_ + pDxz);
-            const float pDyz = cpml_term<float, P>(ps[4], ps[5], rz, oz, zb_, za_, dHx_dz, cp.sc, cp.isc);
-            const float pDyx = cpml_term<float, P>(ps[6], ps[7], rx, ox, xb_, xa_, dHz_dx, cp.sc, cp.isc);
+            const float pDyz = rz ? psi4_step(qyz, e, zb_, za_, dHx_dz, cp.sc, cp.isc) : 0.f;
+            float pDyx = 0.f, pDzx = 0.f;
+            if (rx) {
+                pDyx = psix_step(px.q6[e], ps[rx == 2 ? 7 : 6], px.off[e], px.b[e], px.a[e], dHz_dx, cp.sc, cp.isc);
+                pDzx = psix_step(px.q8[e], ps[rx == 2 ? 9 : 8], px.off[e], px.b[e], px.a[e], dHy_dx, cp.sc, cp.isc);
+            }
             const float cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
-            const float pDzx = cpml_term<float, P>(ps[8], ps[9], rx, ox, xb_, xa_, dHy_dx, cp.sc, cp.isc);
-            const float pDzy = cpml_term<float, P>(ps[10], ps[11], ry, oy, yb_, ya_, dHx_dy, cp.sc, cp.isc);
+            const float pDzy = ry ? psi4_step(qzy, e, yb_, ya_, dHx_dy, cp.sc, cp.isc) : 0.f;
             const float cz_term = (dHy_dx * xk_ + pDzx) - (dHx_dy * yk_ + pDzy);
             const float dxn = el(dx, e) + dt * cx_term;
             const float dyn = el(dy, e) + dt * cy_term;
@@ -504,6 +606,14 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
                 Ey[o + e] = elc(dy, e) * elc(iey4, e);
                 Ez[o + e] = elc(dz, e) * elc(iez4, e);
             }
+        }
+        if (act && ry) {
+            psi4_store(pxy, oy0, qxy);
+            psi4_store(pzy, oy0, qzy);
+        }
+        if (act && rz) {
+            psi4_store(pxz, oz0, qxz);
+            psi4_store(pyz, oz0, qyz);
         }
         hx_p = hx_c;
         hy_p = hy_c;
@@ -900,10 +1010,24 @@ static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
     }
 }
 
-// The vector form needs x rows of a multiple of 4 cells (16-byte aligned rows).
-static bool vec_ok(const mp_yee *g) {
+// The vector form needs x rows of a multiple of 4 cells, the volumes and coefficient lines it
+// reads as float4 on 16-byte boundaries, and the y/z-slab psi arrays on boundaries of four
+// of their elements; anything else takes the scalar kernels.
+static bool al(const void *p, size_t a) { return (uintptr_t)p % a == 0; }
+static bool vec_ok(const mp_yee *g, const mp_cpml *c, bool h) {
     static const bool off = getenv("MP_YEE_SCALAR") != nullptr;
-    return !off && g->nx % VX == 0;
+    if (off || g->nx % VX != 0) return false;
+    const void *v16[] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz, g->dx, g->dy, g->dz,
+                         g->inv_eps_x, g->inv_eps_y, g->inv_eps_z, g->inv_d_cell_x, g->inv_d_dual_x,
+                         c->inv_kappa[0]};
+    for (const void *p : v16)
+        if (!al(p, 16)) return false;
+    if (!al(g->eps_index, 4)) return false;
+    const size_t q = c->psi_dtype == MP_F32 ? 16 : 8;
+    const int yz[] = {0, 1, 2, 3, 4, 5, 10, 11};
+    for (int s : yz)
+        if (!al(h ? c->psi_h[s] : c->psi_d[s], q)) return false;
+    return true;
 }
 
 template <typename S, typename C, typename P>
@@ -947,7 +1071,7 @@ MP_API int mp_update_H_3d(const mp_yee *g, mp_cpml *c, double dt, double mu0) {
     if (rc) return rc;
     if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:528
     const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
-    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g)) {
+    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g, c, true)) {
         if (pd == MP_F32) return launch_H_vec<float>(g, c, dt, mu0);
         if (pd == MP_BF16) return launch_H_vec<bf16>(g, c, dt, mu0);
         if (pd == MP_F16) return launch_H_vec<f16s>(g, c, dt, mu0);
@@ -969,7 +1093,7 @@ MP_API int mp_update_E_3d(const mp_yee *g, mp_cpml *c, double dt) {
     if (rc) return rc;
     if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return MP_OK;  // Kernels.jl:550
     const int32_t fd = g->field_dtype, cd = g->compute_dtype, pd = c->psi_dtype;
-    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g)) {
+    if (fd == MP_F32 && cd == MP_F32 && vec_ok(g, c, false)) {
         if (pd == MP_F32) return launch_E_vec<float>(g, c, dt);
         if (pd == MP_BF16) return launch_E_vec<bf16>(g, c, dt);
         if (pd == MP_F16) return launch_E_vec<f16s>(g, c, dt);
