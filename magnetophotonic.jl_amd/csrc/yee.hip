@@ -314,14 +314,29 @@ __device__ __forceinline__ void psix_load(PsiX<P> &x, P *const *ps, bool load, b
         }
     }
 }
-// _cpml_term_x! (Kernels.jl:16-56) on cell e: psi = fma(b, psi, a*deriv), stored at once
+// _cpml_term_x! (Kernels.jl:16-56) on cell e: psi = fma(b, psi, a*deriv), kept in registers
+__device__ __forceinline__ float psix_step(float &q, float b, float a, float deriv, float, float) {
+    q = fmaT<float>(b, q, a * deriv);
+    return q;
+}
 template <typename P>
-__device__ __forceinline__ float psix_step(P &q, P *arr, int64_t off, float b, float a, float deriv, float sc,
-                                           float isc) {
+__device__ __forceinline__ float psix_step(P &q, float b, float a, float deriv, float sc, float isc) {
     const float p = fmaT<float>(b, psi_val(q, isc), a * deriv);
     psi_put(q, p, sc);
-    arr[off] = q;
     return p;
+}
+// Stores of the x-slab psi after the cell loop: a store inside the loop would make every
+// later wait for a load wait for the store as well (loads and stores share one counter).
+// A skipped cell writes back the bits it loaded.
+template <typename P> __device__ __forceinline__ void psix_store(const PsiX<P> &x, P *const *ps, bool store) {
+    if (!store) return;
+#pragma unroll
+    for (int e = 0; e < VX; ++e) {
+        if (x.r[e]) {
+            ps[x.r[e] == 2 ? 7 : 6][x.off[e]] = x.q6[e];
+            ps[x.r[e] == 2 ? 9 : 8][x.off[e]] = x.q8[e];
+        }
+    }
 }
 
 // The group a lane of a vector kernel updates: either the flattened plane mapping above
@@ -442,8 +457,8 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
         const float pHyz = rz ? psi4_step(qyz, e, zb_, za_, dEx_dz, cp.sc, cp.isc) : 0.f;
         float pHyx = 0.f, pHzx = 0.f;
         if (rx) {
-            pHyx = psix_step(px.q6[e], ps[rx == 2 ? 7 : 6], px.off[e], px.b[e], px.a[e], dEz_dx, cp.sc, cp.isc);
-            pHzx = psix_step(px.q8[e], ps[rx == 2 ? 9 : 8], px.off[e], px.b[e], px.a[e], dEy_dx, cp.sc, cp.isc);
+            pHyx = psix_step(px.q6[e], px.b[e], px.a[e], dEz_dx, cp.sc, cp.isc);
+            pHzx = psix_step(px.q8[e], px.b[e], px.a[e], dEy_dx, cp.sc, cp.isc);
         }
         el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + pHyz) - (dEz_dx * xk_ + pHyx));
         const float pHzy = ry ? psi4_step(qzy, e, yb_, ya_, dEx_dy, cp.sc, cp.isc) : 0.f;
@@ -453,6 +468,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
         st4nt(Hx + o, hx);
         st4nt(Hy + o, hy);
         st4nt(Hz + o, hz);
+        psix_store(px, ps, xslab);
         if (ry) {
             psi4_store(pxy, oy0, qxy);
             psi4_store(pzy, oy0, qzy);
@@ -580,8 +596,8 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
             const float pDyz = rz ? psi4_step(qyz, e, zb_, za_, dHx_dz, cp.sc, cp.isc) : 0.f;
             float pDyx = 0.f, pDzx = 0.f;
             if (rx) {
-                pDyx = psix_step(px.q6[e], ps[rx == 2 ? 7 : 6], px.off[e], px.b[e], px.a[e], dHz_dx, cp.sc, cp.isc);
-                pDzx = psix_step(px.q8[e], ps[rx == 2 ? 9 : 8], px.off[e], px.b[e], px.a[e], dHy_dx, cp.sc, cp.isc);
+                pDyx = psix_step(px.q6[e], px.b[e], px.a[e], dHz_dx, cp.sc, cp.isc);
+                pDzx = psix_step(px.q8[e], px.b[e], px.a[e], dHy_dx, cp.sc, cp.isc);
             }
             const float cy_term = (dHx_dz * zk_ + pDyz) - (dHz_dx * xk_ + pDyx);
             const float pDzy = ry ? psi4_step(qzy, e, yb_, ya_, dHx_dy, cp.sc, cp.isc) : 0.f;
@@ -607,6 +623,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
                 Ez[o + e] = elc(dz, e) * elc(iez4, e);
             }
         }
+        psix_store(px, ps, act && xslab);
         if (act && ry) {
             psi4_store(pxy, oy0, qxy);
             psi4_store(pzy, oy0, qzy);
