@@ -15,8 +15,6 @@
 // written there, compiled with -ffp-contract=off.
 #include <algorithm>
 #include <cstdlib>
-#include <mutex>
-#include <vector>
 
 #include "common.h"
 
@@ -339,26 +337,66 @@ template <typename P> __device__ __forceinline__ void psix_store(const PsiX<P> &
     }
 }
 
+// Layout of the CPML shell around the interior box, in memory order: the mz low z planes
+// whole; then in each of the nk box planes the my low rows whole, the x strips of the nj box
+// rows (groups [0, gi0) and [gi0+ngi, G)) and the my high rows whole; then the mz high z
+// planes whole.  Shell entry t is decoded arithmetically from these counts, so a lane's
+// first loads depend on no load of its own.  Divisions by the uniform counts use the
+// rounded-up reciprocal: q = umulhi(n, ceil(2^32/d)) is n/d or one more for n < 2^32.
+struct ShellGeom {
+    uint32_t n;                          // shell groups (0: plane mapping instead)
+    uint32_t G, ny, nz, gi0, ngi, my, nj, mz;
+    uint32_t pfull, pint, gx, nlo, nmid;   // groups per whole plane / per box plane, strip groups per row
+    uint32_t r_pfull, r_pint, r_g, r_gx;   // ceil(2^32 / d) of those divisors
+};
+__device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t d, uint32_t r) {
+    uint32_t q = __umulhi(n, r);
+    if ((uint64_t)q * d > n) --q;
+    return q;
+}
+__device__ __forceinline__ void shell_pos(const ShellGeom &sg, uint32_t t, uint32_t &g, uint32_t &j, uint32_t &k) {
+    if (t < sg.nlo || t >= sg.nlo + sg.nmid) {   // a whole z plane
+        const uint32_t u = t < sg.nlo ? t : t - sg.nlo - sg.nmid;
+        const uint32_t kk = udiv(u, sg.pfull, sg.r_pfull), r = u - kk * sg.pfull;
+        k = t < sg.nlo ? kk : sg.nz - sg.mz + kk;
+        j = udiv(r, sg.G, sg.r_g);
+        g = r - j * sg.G;
+        return;
+    }
+    const uint32_t u = t - sg.nlo;
+    const uint32_t kk = udiv(u, sg.pint, sg.r_pint);
+    uint32_t r = u - kk * sg.pint;
+    k = sg.mz + kk;
+    const uint32_t lo = sg.my * sg.G, strips = sg.nj * sg.gx;
+    if (r < lo || r >= lo + strips) {   // a whole low or high row
+        const bool hi = r >= lo;
+        if (hi) r -= lo + strips;
+        const uint32_t jj = udiv(r, sg.G, sg.r_g);
+        j = hi ? sg.ny - sg.my + jj : jj;
+        g = r - jj * sg.G;
+    } else {   // an x strip of a box row
+        r -= lo;
+        const uint32_t jj = udiv(r, sg.gx, sg.r_gx), gg = r - jj * sg.gx;
+        j = sg.my + jj;
+        g = gg < sg.gi0 ? gg : gg + sg.ngi;
+    }
+}
+
 // The group a lane of a vector kernel updates: either the flattened plane mapping above
-// (the whole volume, one z-plane per workgroup) or entry t of a list of groups (the CPML
-// shell; entry q is the group whose first cell is 4q).  in_range is false for lanes past
-// the end; they still load from valid addresses and take part in the shuffles.
+// (the whole volume, one z-plane per workgroup) or shell entry t.  in_range is false for
+// lanes past the end; they still load from valid addresses and take part in the shuffles.
 struct LaneGroup { int64_t i0, j, k; bool in_range; };
-__device__ __forceinline__ LaneGroup lane_group(const uint32_t *__restrict__ shell, int64_t nshell, int64_t nx,
-                                                int64_t ny, int64_t bpp, int64_t nbz, bool remap, int tyv) {
+__device__ __forceinline__ LaneGroup lane_group(const ShellGeom &sg, int64_t nx, int64_t ny, int64_t bpp, int64_t nbz,
+                                                bool remap, int tyv) {
     LaneGroup r;
-    if (shell) {
-        // list entries, rows and planes fit 32 bits (the launcher checks), so the index
-        // arithmetic stays in 32-bit integers
-        const int64_t t = (int64_t)blockIdx.x * (64 * tyv) + threadIdx.y * 64 + threadIdx.x;
-        r.in_range = t < nshell;
-        const uint32_t q = r.in_range ? shell[t] : 0u;
-        const uint32_t G = (uint32_t)(nx / VX), ny32 = (uint32_t)ny;
-        const uint32_t rest = q / G;
-        const uint32_t kq = rest / ny32;
-        r.i0 = (int64_t)(q - rest * G) * VX;
-        r.j = (int64_t)(rest - kq * ny32);
-        r.k = (int64_t)kq;
+    if (sg.n) {
+        const uint32_t t = blockIdx.x * (uint32_t)(64 * tyv) + threadIdx.y * 64 + threadIdx.x;
+        r.in_range = t < sg.n;
+        uint32_t g, j, k;
+        shell_pos(sg, r.in_range ? t : 0u, g, j, k);
+        r.i0 = (int64_t)g * VX;
+        r.j = (int64_t)j;
+        r.k = (int64_t)k;
     } else {
         const PlaneGroup pg = plane_group(nx, ny, bpp, nbz, remap, tyv);
         r.i0 = pg.i0;
@@ -386,9 +424,9 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
     const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, int64_t nbz, int remap,
-    const uint32_t *__restrict__ shell, int64_t nshell) {
+    ShellGeom sg) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
-    const LaneGroup lg = lane_group(shell, nshell, nx, ny, nbx, nbz, remap != 0, TYV);
+    const LaneGroup lg = lane_group(sg, nx, ny, nbx, nbz, remap != 0, TYV);
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
     const bool act = lg.in_range && lg.j < ny - 1 && lg.k < nz - 1;
     const int64_t k = act ? lg.k : 0;
@@ -491,7 +529,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
     const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, int64_t nby,
-    int64_t nbz, int remap, const uint32_t *__restrict__ shell, int64_t nshell) {
+    int64_t nbz, int remap, ShellGeom sg) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
     __shared__ float4 tab[MI ? 256 : 1];
     if (MI) {
@@ -503,7 +541,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     bool act;
     if (LIST) {
         // one listed group per lane, one plane; every lane stays for the shuffles
-        const LaneGroup lg = lane_group(shell, nshell, nx, ny, 1, 1, false, TYV);
+        const LaneGroup lg = lane_group(sg, nx, ny, 1, 1, false, TYV);
         act = lg.in_range && lg.j >= 1 && lg.k >= 1;
         i0 = lg.i0;
         j = lg.j;
@@ -537,14 +575,12 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
         const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
         const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
         float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
+        // inv_eps (or the material bytes) load with the fields; the table lookup follows the
+        // cell loop, so no load waits before the last one is issued
         float4 iex4, iey4, iez4;
+        uint32_t m4 = 0u;
         if (MI) {
-            const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
-            const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
-            const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
-            iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
-            iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
-            iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+            m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
         } else {
             iex4 = ld4nt(iex + o);
             iey4 = ld4nt(iey + o);
@@ -609,6 +645,13 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
             el(dy, e) = dyn;
             el(dz, e) = dzn;
         }
+        if (MI) {
+            const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+            const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+            iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+            iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+            iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+        }
         // E = D * inv_eps, formed after the loop so E does not occupy registers through it
         if (act && ic > 0) {
             st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
@@ -644,8 +687,8 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
 // terms added as +0 (bit for bit what the general kernels compute there) without the slab
 // tests, psi offsets and skip branches, in far fewer registers.  Lanes take consecutive
 // groups of the box, row after row and plane after plane, so a wave's loads are contiguous
-// runs.  The CPML shell around the box is updated by the general kernels from a list of its
-// groups; no cache line is written by both launches.
+// runs.  The CPML shell around the box is updated by the general kernels, whose lanes decode
+// their shell group from the shell's layout; no cache line is written by both launches.
 __device__ __forceinline__ void box_pos(uint32_t t, uint32_t per_plane, uint32_t ngi, uint32_t &kk, uint32_t &jj,
                                         uint32_t &gg) {
     kk = t / per_plane;
@@ -732,19 +775,6 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
     const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
     const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
     float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
-    float4 iex4, iey4, iez4;
-    if (MI) {
-        const uint32_t m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
-        const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
-        const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
-        iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
-        iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
-        iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
-    } else {
-        iex4 = ld4nt(iex + o);
-        iey4 = ld4nt(iey + o);
-        iez4 = ld4nt(iez + o);
-    }
     // the previous lane holds the previous group of this row unless this is its first group
     float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
     if ((threadIdx.x & 63) == 0 || gg == 0) {
@@ -753,6 +783,17 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
     }
     const float4 idx4 = ld4(iddx + i0), xk4 = ld4(xk + i0);
     const float idy_ = iddy[j], yk_ = yk[j], idz_ = iddz[k], zk_ = zk[k];
+    // inv_eps (or the material bytes) load with the fields; the table lookup follows the cell
+    // loop, so no load waits before the last one is issued
+    float4 iex4, iey4, iez4;
+    uint32_t m4 = 0u;
+    if (MI) {
+        m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
+    } else {
+        iex4 = ld4nt(iex + o);
+        iey4 = ld4nt(iey + o);
+        iez4 = ld4nt(iez + o);
+    }
 #pragma unroll
     for (int e = 0; e < VX; ++e) {
         const float hxc = elc(hx_c, e), hyc = elc(hy_c, e), hzc = elc(hz_c, e);
@@ -771,6 +812,13 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
         el(dx, e) = el(dx, e) + dt * cx_term;
         el(dy, e) = el(dy, e) + dt * cy_term;
         el(dz, e) = el(dz, e) + dt * cz_term;
+    }
+    if (MI) {
+        const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+        const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+        iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+        iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+        iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
     }
     if (act) {
         st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
@@ -860,59 +908,41 @@ static Box interior_box(const mp_yee *g, const mp_cpml *c) {
 }
 
 // The H and E updates split into interior and shell launches when the box is non-empty and
-// every group index fits 32 bits (MP_YEE_NO_SPLIT=1 keeps one launch over the whole volume).
+// every group index, lanes past the end included, fits 32 bits (MP_YEE_NO_SPLIT=1 keeps one
+// launch over the whole volume).
 static bool split_ok(const mp_yee *g, const Box &b) {
     static const bool off = getenv("MP_YEE_NO_SPLIT") != nullptr;
-    return !off && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32);
+    return !off && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32) - 4096;
 }
 
-// Device list of the shell groups (every group of the volume outside the box, in memory
-// order), built on the host once per device and grid shape and kept until the process ends.
-struct ShellList {
-    int dev;
-    int64_t nx, ny, nz, gi0, ngi, j0, nj, k0, nk;
-    uint32_t *d;
-    int64_t n;
-};
-static std::vector<ShellList> g_shells;
-static std::mutex g_shell_mu;
-
-static int shell_list(const mp_yee *g, const Box &b, const uint32_t **out, int64_t *nout) {
-    int dev = 0;
-    MP_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(g_shell_mu);
-    for (const ShellList &s : g_shells) {
-        if (s.dev == dev && s.nx == g->nx && s.ny == g->ny && s.nz == g->nz && s.gi0 == b.gi0 && s.ngi == b.ngi &&
-            s.j0 == b.j0 && s.nj == b.nj && s.k0 == b.k0 && s.nk == b.nk) {
-            *out = s.d;
-            *nout = s.n;
-            return MP_OK;
-        }
-    }
+// Shell layout of a box (see ShellGeom); n = 0 when a divisor would be below 2, where the
+// launchers keep the whole-volume launch.
+static ShellGeom shell_geom(const mp_yee *g, const Box &b) {
+    ShellGeom sg{};
     const int64_t G = g->nx / VX, ny = g->ny, nz = g->nz;
-    std::vector<uint32_t> h;
-    h.reserve((size_t)(G * ny * nz - b.n()));
-    for (int64_t k = 0; k < nz; ++k) {
-        const bool kin = k >= b.k0 && k < b.k0 + b.nk;
-        for (int64_t j = 0; j < ny; ++j) {
-            const int64_t base = G * (j + ny * k);
-            if (kin && j >= b.j0 && j < b.j0 + b.nj) {
-                for (int64_t q = 0; q < b.gi0; ++q) h.push_back((uint32_t)(base + q));
-                for (int64_t q = b.gi0 + b.ngi; q < G; ++q) h.push_back((uint32_t)(base + q));
-            } else {
-                for (int64_t q = 0; q < G; ++q) h.push_back((uint32_t)(base + q));
-            }
-        }
-    }
-    ShellList s{dev, g->nx, g->ny, g->nz, b.gi0, b.ngi, b.j0, b.nj, b.k0, b.nk, nullptr, (int64_t)h.size()};
-    if (s.n > 0) {
-        MP_HIP(hipMalloc((void **)&s.d, h.size() * sizeof(uint32_t)));
-        MP_HIP(hipMemcpy(s.d, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    }
-    g_shells.push_back(s);
-    *out = s.d;
-    *nout = s.n;
-    return MP_OK;
+    const int64_t pfull = ny * G, gx = G - b.ngi, pint = 2 * b.j0 * G + b.nj * gx;
+    const int64_t nlo = b.k0 * pfull, nmid = b.nk * pint, n = 2 * nlo + nmid;
+    if (pfull < 2 || pint < 2 || G < 2 || gx < 2 || n != G * ny * nz - b.n()) return sg;
+    auto recip = [](int64_t d) { return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d); };
+    sg.n = (uint32_t)n;
+    sg.G = (uint32_t)G;
+    sg.ny = (uint32_t)ny;
+    sg.nz = (uint32_t)nz;
+    sg.gi0 = (uint32_t)b.gi0;
+    sg.ngi = (uint32_t)b.ngi;
+    sg.my = (uint32_t)b.j0;
+    sg.nj = (uint32_t)b.nj;
+    sg.mz = (uint32_t)b.k0;
+    sg.pfull = (uint32_t)pfull;
+    sg.pint = (uint32_t)pint;
+    sg.gx = (uint32_t)gx;
+    sg.nlo = (uint32_t)nlo;
+    sg.nmid = (uint32_t)nmid;
+    sg.r_pfull = recip(pfull);
+    sg.r_pint = recip(pint);
+    sg.r_g = recip(G);
+    sg.r_gx = recip(gx);
+    return sg;
 }
 
 template <typename P, int TY>
@@ -921,8 +951,9 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:60-101)
     const float dt_mu0 = (float)dt / (float)mu0;
     const Box b = interior_box(g, c);
+    const ShellGeom sg = split_ok(g, b) ? shell_geom(g, b) : ShellGeom{};
     dim3 block(64, TY, 1);
-    if (!split_ok(g, b)) {
+    if (!sg.n) {
         // one launch: one z-plane per workgroup range (nbx workgroups per plane)
         const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
         KernelTimer t("yee_H");
@@ -930,13 +961,10 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
             (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
             (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
             (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, nbz, xcd_remap(),
-            nullptr, 0);
+            ShellGeom{});
         MP_LAUNCH_CHECK("yee_H_vec_kernel");
         return MP_OK;
     }
-    const uint32_t *shell = nullptr;
-    int64_t nshell = 0;
-    if (int e = shell_list(g, b, &shell, &nshell)) return e;
     KernelTimer t("yee_H");
     yee_H_int_kernel<<<cdiv(b.n(), 256), 256, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
@@ -945,13 +973,11 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         (const float *)c->inv_kappa[2], nx, ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0,
         (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0, (uint32_t)b.n(), dt_mu0);
     MP_LAUNCH_CHECK("yee_H_int_kernel");
-    if (nshell > 0) {
-        yee_H_vec_kernel<P, TY><<<cdiv(nshell, 64 * TY), block, 0, stream()>>>(
-            (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
-            (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, 1, 0, shell, nshell);
-        MP_LAUNCH_CHECK("yee_H_vec_kernel");
-    }
+    yee_H_vec_kernel<P, TY><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
+        (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+        (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, 1, 0, sg);
+    MP_LAUNCH_CHECK("yee_H_vec_kernel");
     return MP_OK;
 }
 
@@ -962,6 +988,7 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
                     getenv("MP_YEE_NO_EPS_INDEX") == nullptr;
     const Box b = interior_box(g, c);
+    const ShellGeom sg = split_ok(g, b) ? shell_geom(g, b) : ShellGeom{};
     dim3 block(64, TY, 1);
 #define E_ARGS                                                                                             \
     (float *)g->ex, (float *)g->ey, (float *)g->ez, (const float *)g->hx, (const float *)g->hy,            \
@@ -969,23 +996,20 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
         (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, g->eps_index,                            \
         (const float4 *)g->eps_table, g->n_eps, (const float *)g->inv_d_dual_x,                            \
         (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z
-    if (!split_ok(g, b)) {
+    if (!sg.n) {
         // one launch: one z-plane per workgroup range (nbx workgroups per plane)
         const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
         const unsigned grid = (unsigned)(nbx * nbz);
         KernelTimer t("yee_E");
         if (mi)
             yee_E_vec_kernel<P, TY, true, false><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), nullptr, 0);
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), ShellGeom{});
         else
             yee_E_vec_kernel<P, TY, false, false><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), nullptr, 0);
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), ShellGeom{});
         MP_LAUNCH_CHECK("yee_E_vec_kernel");
         return MP_OK;
     }
-    const uint32_t *shell = nullptr;
-    int64_t nshell = 0;
-    if (int e = shell_list(g, b, &shell, &nshell)) return e;
     KernelTimer t("yee_E");
 #define INT_ARGS                                                                                          \
     (const float *)c->inv_kappa[0], (const float *)c->inv_kappa[1], (const float *)c->inv_kappa[2], nx, ny, \
@@ -994,16 +1018,14 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     if (mi) yee_E_int_kernel<true><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
     else yee_E_int_kernel<false><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
     MP_LAUNCH_CHECK("yee_E_int_kernel");
-    if (nshell > 0) {
-        const unsigned grid = cdiv(nshell, 64 * TY);
-        if (mi)
-            yee_E_vec_kernel<P, TY, true, true><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, shell, nshell);
-        else
-            yee_E_vec_kernel<P, TY, false, true><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, shell, nshell);
-        MP_LAUNCH_CHECK("yee_E_vec_kernel");
-    }
+    const unsigned grid = cdiv(sg.n, 64 * TY);
+    if (mi)
+        yee_E_vec_kernel<P, TY, true, true><<<grid, block, 0, stream()>>>(
+            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, sg);
+    else
+        yee_E_vec_kernel<P, TY, false, true><<<grid, block, 0, stream()>>>(
+            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, sg);
+    MP_LAUNCH_CHECK("yee_E_vec_kernel");
 #undef INT_ARGS
 #undef E_ARGS
     return MP_OK;
