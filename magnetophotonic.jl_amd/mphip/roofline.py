@@ -107,7 +107,7 @@ def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch:
 # KiB per dispatch; the factors are what the counters read for a plain copy of known size
 # on gfx950), in runs separate from the timed one: profiles/yee_E_coupled_pump_512x512x256_pmc.md.
 MEASURED_TRAFFIC = {
-    ("coupled_pump_512x512x256", "plain", "index"): {"yee_E": 4166928390, "yee_H": 3277202058},
+    ("coupled_pump_512x512x256", "plain", "index"): {"yee_E": 4133190159, "yee_H": 3222221392},
 }
 
 

@@ -323,11 +323,12 @@ def test_fused_film_epilogue_matches_separate_kernels():
     sep.free()
 
 
-@pytest.mark.parametrize("shape,npml", [((96, 68, 64), 6), ((128, 40, 72), 9)])
+@pytest.mark.parametrize("shape,npml", [((96, 68, 64), 6), ((128, 40, 72), 9), ((256, 64, 48), 1)])
 def test_split_yee_update_bitwise(shape, npml):
-    """Grids whose H update splits into the interior-box launch and the CPML-shell list
-    launch (box edges on whole cache lines, shells of uneven width, rows that are not a
-    multiple of the wave): every field and psi slab bit-identical to the oracle."""
+    """Grids whose H and E updates split into the interior-box launch and the CPML-shell
+    launch, whose lanes decode their group from the shell layout (box edges on whole cache
+    lines, shells of uneven and of one-cell width, rows that are not a multiple of the wave,
+    lanes past the end of the shell): every field and psi slab bit-identical to the oracle."""
     import driver
     from mphip import configs
     from mphip.state import FDTDState
