@@ -3,12 +3,18 @@
 // Reference: src/fdtd/Kernels.jl:9-152 (yee_H_3d_kernel!, yee_E_3d_kernel!,
 // _cpml_term_{x,y,z}!, _pml_slab) and the launchers :522-565.
 //
-// Layout: volumes column-major, x fastest.  A workgroup is a 64 x 4 tile of (i, j)
-// columns (one wave64 per x-row, so every global access of a wave is 64 consecutive
-// elements = 256 B for FP32); each thread marches a chunk of z planes and keeps the
-// previous plane's values in registers, so the z-neighbour never comes from memory
-// twice.  x/y neighbours are re-read from the same or the adjacent rows and are served
-// by the L1/L2 caches.  CPML psi slabs are touched only by threads inside a slab.
+// Layout: volumes column-major, x fastest.  Three kernel families:
+//  - FP32 with rows of a multiple of 4 cells (the vector form): each lane owns a float4
+//    group of 4 x-consecutive cells.  The interior box (no CPML slab, no boundary cell)
+//    is updated by lean kernels without psi terms; the CPML shell around it by the
+//    general vector kernels, whose lanes decode their group from the shell layout and
+//    load every psi value before the cell loop.  Where the box is empty the general
+//    kernels cover the whole volume, a z-plane per workgroup range.
+//  - Other shapes and FP64: the scalar kernels.  A workgroup is a 64 x 4 tile of (i, j)
+//    columns (one wave64 per x-row, so every global access of a wave is 64 consecutive
+//    elements); each thread marches a chunk of z planes and keeps the previous plane's
+//    values in registers, so the z-neighbour never comes from memory twice.
+// CPML psi slabs are touched only by lanes inside a slab.
 //
 // Arithmetic order is the reference's: derivative = (E[n] - E[c]) * inv_d, psi update
 // = fma(b, psi, a*deriv) (Julia muladd), curl terms and the D/H updates exactly as
