@@ -307,6 +307,10 @@ int mp_event_elapsed_ms(void *start, void *stop, float *ms);  /* synchronises on
  * mp_kernel_stats (names are the kernel names below).  Off by default. */
 int mp_profile_enable(int on);
 int mp_kernel_stats(const char *name, int64_t *launches, double *total_ms);
+/* Restrict the per-kernel timing to a comma-separated list of kernel names (NULL or ""
+ * times every kernel), so that a timed run brackets only the launches it reports.  No
+ * counterpart in the reference (instrumentation of this library). */
+int mp_profile_select(const char *names);
 
 /* ---- hot-path kernels ---------------------------------------------------------------- */
 

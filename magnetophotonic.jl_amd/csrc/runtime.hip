@@ -16,6 +16,7 @@ namespace mp {
 static thread_local char g_err[512] = {0};
 static hipStream_t g_stream = nullptr;
 static bool g_profile = false;
+static std::vector<std::string> g_profile_only;   // empty: every kernel name is timed
 
 struct PendingTiming {
     const char *name;
@@ -80,8 +81,15 @@ static void drain_pending() {
     g_pending.clear();
 }
 
+static bool profiled(const char *name) {
+    if (g_profile_only.empty()) return true;
+    for (const std::string &n : g_profile_only)
+        if (n == name) return true;
+    return false;
+}
+
 KernelTimer::KernelTimer(const char *name) : name_(name) {
-    if (!g_profile) return;
+    if (!g_profile || !profiled(name)) return;
     hipEvent_t e = pool_event();
     if (e && hipEventRecord(e, g_stream) == hipSuccess) start_ = (void *)e;
 }
@@ -217,6 +225,23 @@ MP_API int mp_profile_enable(int on) {
     drain_pending();
     if (on) g_stats.clear();
     g_profile = on != 0;
+    return MP_OK;
+}
+
+MP_API int mp_profile_select(const char *names) {
+    drain_pending();
+    g_profile_only.clear();
+    if (!names) return MP_OK;
+    std::string cur;
+    for (const char *c = names;; ++c) {
+        if (*c == ',' || *c == '\0') {
+            if (!cur.empty()) g_profile_only.push_back(cur);
+            cur.clear();
+            if (*c == '\0') break;
+        } else {
+            cur.push_back(*c);
+        }
+    }
     return MP_OK;
 }
 

@@ -154,6 +154,7 @@ RUNTIME_SIGNATURES = {
     "mp_event_elapsed_ms": (C.c_int, [vp, vp, P(C.c_float)]),
     "mp_profile_enable": (C.c_int, [C.c_int]),
     "mp_kernel_stats": (C.c_int, [C.c_char_p, P(i64), P(f64)]),
+    "mp_profile_select": (C.c_int, [C.c_char_p]),
     "mp_any_nonfinite": (C.c_int, [vp, i32, i64, vp]),
 }
 
