@@ -28,15 +28,19 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
 
 
-@pytest.mark.parametrize("kind", ["bf16", "f16"])
-def test_reduced_storage_bitwise(kind):
+# 40 x 36 x 64 runs the whole-volume Yee launches; 96 x 68 x 64 the interior-box and
+# CPML-shell launches (the shell's psi in the reduced storage)
+@pytest.mark.parametrize("kind,shape", [("bf16", (40, 36, 64)), ("f16", (40, 36, 64)),
+                                        ("bf16", (96, 68, 64)), ("f16", (96, 68, 64))])
+def test_reduced_storage_bitwise(kind, shape):
     import driver
     from mphip import configs
     from mphip.state import FDTDState
-    spec, _ = configs.coupled_pump(nx=40, ny=36, nz=64, steps=200, fluence_mj_cm2=12.0, npml=6,
+    nx, ny, nz = shape
+    spec, _ = configs.coupled_pump(nx=nx, ny=ny, nz=nz, steps=200, fluence_mj_cm2=12.0, npml=6,
                                    src_z=40e-9, n_planes=0, enable_mo=False, psi_precision=kind,
                                    ade_state_precision=kind)
-    print(f"case reduced storage bitwise {kind} 40x36x64", flush=True)
+    print(f"case reduced storage bitwise {kind} {nx}x{ny}x{nz}", flush=True)
     gpu = FDTDState(spec)
     ref = driver.OracleState(spec)
     gpu.run(200)
