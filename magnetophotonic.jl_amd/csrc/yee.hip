@@ -319,10 +319,6 @@ __device__ __forceinline__ void psix_load(PsiX<P> &x, P *const *ps, bool load, b
     }
 }
 // _cpml_term_x! (Kernels.jl:16-56) on cell e: psi = fma(b, psi, a*deriv), kept in registers
-__device__ __forceinline__ float psix_step(float &q, float b, float a, float deriv, float, float) {
-    q = fmaT<float>(b, q, a * deriv);
-    return q;
-}
 template <typename P>
 __device__ __forceinline__ float psix_step(P &q, float b, float a, float deriv, float sc, float isc) {
     const float p = fmaT<float>(b, psi_val(q, isc), a * deriv);
