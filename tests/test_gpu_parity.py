@@ -164,6 +164,49 @@ def test_solved_mode_source_bitwise():
     gpu.free()
 
 
+def test_mode_source_dft_record_batched():
+    """A mode source also writes H, after the H update.  The DFT record that mp_run_steps
+    batches (snapshots, then one pass per batch) must see that H, as the oracle's per-step
+    record does; a plane on the source plane itself pins the order."""
+    import math
+    import driver
+    from mphip import geometry, grid, sources
+    from mphip.modesolver import mode_source
+    from mphip.monitors import DFTPlanes
+    from mphip.spec import build_state_spec
+    from mphip.state import FDTDState
+    d = 20e-9
+    g = grid.Grid3D(grid.uniform_axis_n(48, d), grid.uniform_axis_n(40, d), grid.uniform_axis_n(36, d))
+    sc = geometry.Scene()
+    sc.add(geometry.Box(0.0, 48 * d, 14 * d, 26 * d, 13 * d, 23 * d), geometry.Material("core", epsr=4.0))
+    geo = geometry.rasterize(sc, g, background=1.44 ** 2)
+    w0 = 2 * math.pi * grid.C0 / 800e-9
+    pulse = sources.GaussianPulse(amplitude=1e3, t0=4e-15, tau=1.5e-15, omega=w0)
+    src = mode_source(g, pulse, 8, 800e-9, epsr_volume=geo.epsr, component="Ez")
+    spec = build_state_spec(g, geo, grid.cfl_dt_cells(g, courant=0.5), source=src, n_pml=4)
+    planes = [(0, 8), (0, 30), (1, 20), (2, 18)]
+    bins = [w0 * f for f in (0.97, 1.0, 1.03)]
+    print("case mode source + batched DFT record 48x40x36", flush=True)
+    gpu, ref = FDTDState(spec), driver.OracleState(spec)
+    mon = DFTPlanes(gpu, planes, bins)
+    gpu.attach_dft(mon)
+    ref_mon = OracleDFT(ref, planes, bins)
+    gpu.run(45)
+    for _ in range(45):
+        ref.step()
+        ref_mon.record(ref)
+    gpu.synchronize()
+    for name in FIELDS:
+        assert np.array_equal(gpu.field(name), ref.field(name)), name
+    for p in range(len(planes)):
+        gm, rm = mon.download(p), ref_mon.download(p)
+        for k in ("ea", "eb", "ha", "hb", "energy"):
+            # fields are bit-identical; cos/sin of the phase factors differ in the last ULP
+            assert rel_l2(gm[k], rm[k]) < 1e-10, (planes[p], k, rel_l2(gm[k], rm[k]))
+        assert np.abs(rm["hb"]).max() > 0 or np.abs(rm["ha"]).max() > 0
+    gpu.free()
+
+
 def test_relax_and_reductions():
     import driver
     from mphip import configs
