@@ -29,7 +29,7 @@ METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Î”m|â
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "film_epilogue", "multiphysics",
            "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate", "dft_snapshot", "dft_batch")
-DFT_BATCH = int(os.environ.get("MP_DFT_BATCH", "32"))
+DFT_BATCH = 32   # steps per batched DFT pass in the library
 
 
 def build_workload(name: str, form: str):
@@ -158,7 +158,7 @@ def main():
 
     nx, ny, nz = spec.shape
     ncell = nx * ny * nz
-    eps_idx = st.yee.n_eps > 0 and os.environ.get("MP_YEE_NO_EPS_INDEX") is None
+    eps_idx = st.yee.n_eps > 0
     kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH, st.fused_epilogue)
     if "multiphysics" in kb:
         kb["surrogate"] = kb["multiphysics"]

@@ -72,8 +72,17 @@ typedef struct mp_yee {
     const uint8_t *eps_index;
     const float *eps_table;
     int32_t n_eps;
-    int32_t _pad_eps;
+    /* Kernel family of the FP32 updates (enum mp_yee_form): MP_YEE_AUTO picks the
+     * interior/shell split where it applies; the other values force one form so that each
+     * can be checked against the others.  Every form computes the same values. */
+    int32_t kernel_form;
 } mp_yee;
+
+enum mp_yee_form {
+    MP_YEE_AUTO = 0,    /* interior + CPML-shell launches where the box is non-empty */
+    MP_YEE_WHOLE = 1,   /* one vector launch over the whole volume */
+    MP_YEE_SCALAR = 2   /* the scalar (one cell per lane) kernels */
+};
 
 /* CFS-CPML state (src/fdtd/CPML.jl:79-189).  Per-axis coefficient vectors inv_kappa,
  * a, b (length n of that axis, CT) and the 24 convolution-memory slabs: an x-slab is
@@ -268,7 +277,10 @@ typedef struct mp_state {
      * cell n) and the ADE state is stored in the compute type: the step then runs the
      * list work after the E update as one mp_film_epilogue pass. */
     int32_t film_lists_coincide;
-    int32_t _pad_state;
+    /* Non-zero runs the film update on the library stream in step order; zero (default)
+     * runs it on a side stream beside the next step's Yee updates, joined before the
+     * next gyration / absorbed-power kernels, which gives the same values. */
+    int32_t film_serial;
 } mp_state;
 
 /* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */

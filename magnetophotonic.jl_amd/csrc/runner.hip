@@ -51,8 +51,6 @@ struct Overlap {
     hipStream_t side = nullptr;
     hipEvent_t main_done = nullptr, film_done = nullptr;
     bool pending = false;     // film_done recorded and not yet waited on
-    bool deferred = false;    // a film update due but not yet launched
-    double deferred_dt = 0.0;
 };
 static Overlap g_ov;
 
@@ -69,24 +67,6 @@ static int overlap_join() {
     MP_HIP(hipStreamWaitEvent(stream(), g_ov.film_done, 0));
     g_ov.pending = false;
     return MP_OK;
-}
-
-static bool overlap_enabled() {
-    static const bool on = getenv("MP_NO_OVERLAP") == nullptr;
-    return on;
-}
-
-// With overlap, a film update that falls due at the end of step n is launched at once, so
-// it runs beside step n+1's H update.  MP_FILM_AFTER_H=1 launches it after that H update
-// instead, beside the E update (nothing between the two points touches film state, and
-// the main stream still waits for it before step n+1's gyration).  On the 512x512x256
-// step the E update suffers more from the company: 2.28 ms per step against 2.19 ms.
-static bool film_after_h() {
-    static const bool on = [] {
-        const char *e = getenv("MP_FILM_AFTER_H");
-        return e && e[0] == '1';
-    }();
-    return on;
 }
 
 static int film_update(mp_state *s, double dt_mp) {
@@ -110,19 +90,12 @@ static int launch_film_side(mp_state *s, double dt_mp) {
     return MP_OK;
 }
 
-static int launch_deferred(mp_state *s) {
-    if (!g_ov.deferred) return MP_OK;
-    g_ov.deferred = false;
-    return launch_film_side(s, g_ov.deferred_dt);
-}
-
 static int film_list_work(mp_state *s);
 static int film_update_due(mp_state *s);
 
 static int one_step(mp_state *s) {
     int rc;
     if ((rc = mp_update_H_3d(s->yee, s->cpml, s->dt, s->mu0))) return rc;
-    if ((rc = launch_deferred(s))) return rc;   // the previous step's film update, beside this E update
     if ((rc = inject(s))) return rc;
     if ((rc = mp_update_E_3d(s->yee, s->cpml, s->dt))) return rc;
     if (s->film_lists_coincide && s->ade[0] && s->film) {
@@ -172,6 +145,9 @@ static int film_list_work(mp_state *s) {
 }
 
 // The 4TM + LLB update when the multiphysics window closes at step n (Solver.jl:356-367).
+// Launched at once: on the side stream it runs beside step n+1's H update (launching it
+// after that H update instead, beside the E update, measured 2.28 against 2.19 ms per
+// 512x512x256 step: the E update suffers more from the company).
 static int film_update_due(mp_state *s) {
     int rc;
     if (s->film) {
@@ -179,11 +155,8 @@ static int film_update_due(mp_state *s) {
             const int64_t window = s->n - s->last_mp_n;
             if (window > 0) {
                 const double dt_mp = s->dt * (double)window;
-                if (!overlap_enabled()) {
+                if (s->film_serial) {
                     if ((rc = film_update(s, dt_mp))) return rc;
-                } else if (film_after_h()) {
-                    g_ov.deferred = true;
-                    g_ov.deferred_dt = dt_mp;
                 } else if ((rc = launch_film_side(s, dt_mp))) {
                     return rc;
                 }
@@ -202,15 +175,9 @@ MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
         int rc = one_step(s);
         if (rc) {
             if (s->record_dft && s->nplanes > 0) dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt);
-            launch_deferred(s);
             overlap_join();
             return rc;
         }
-    }
-    // a film update still due is issued before the call returns
-    if (int rc = launch_deferred(s)) {
-        overlap_join();
-        return rc;
     }
     // DFT steps still held in the batch reach the accumulators before the call returns
     if (s->record_dft && s->nplanes > 0) {

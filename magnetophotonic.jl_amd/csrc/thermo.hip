@@ -128,17 +128,12 @@ __global__ __launch_bounds__(128) void multiphysics_kernel(
   }
 }
 
-// Workgroups of the film update: at most 1024 (MP_FILM_GRID overrides; 0 = one per 128
-// cells).  The kernel loops over the cells; on the 2.6 M-cell film a 1024-workgroup grid
-// overlaps the Yee updates of the next step with 1% less step time than a full grid.
+// Workgroups of the film update: at most 1024.  The kernel loops over the cells; on the
+// 2.6 M-cell film a 1024-workgroup grid overlaps the Yee updates of the next step with 1%
+// less step time than a full grid (512 was slower, 768 / 1280 / 2048 no faster).
 static int64_t film_grid(int64_t N) {
-    static int64_t cap = -1;
-    if (cap < 0) {
-        const char *e = getenv("MP_FILM_GRID");
-        cap = e ? atoll(e) : 1024;
-    }
     const int64_t full = cdiv(N, 128);
-    return cap > 0 && cap < full ? cap : full;
+    return full < 1024 ? full : 1024;
 }
 
 template <typename L>

@@ -234,20 +234,13 @@ __device__ __forceinline__ void st4nt(float *p, float4 v) {
     __builtin_nontemporal_store(w, reinterpret_cast<v4f *>(p));
 }
 
-// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8 XCDs, each with
-// its own L2; remapping the dispatch index gives every XCD a contiguous range of tiles,
-// so a tile and its y neighbour (which shares a halo row) usually meet in the same L2.
-constexpr int NXCD = 8;
 // Flattened plane mapping of the vector kernels: consecutive rows of a z-plane are
 // adjacent in memory, so a plane is one run of nx*ny/4 float4 groups, row after row.
 // Lanes take consecutive groups across row ends (no idle lanes when a row is not a
 // multiple of 256 cells) and every load of a wave stays contiguous.
 struct PlaneGroup { int64_t i0, j, bz; bool in_plane; };
-__device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_t bpp, int64_t nbz, bool remap,
-                                                  int tyv) {
-    const int64_t nb = bpp * nbz;
-    const int64_t b = blockIdx.x;
-    const int64_t t = (remap && nb % NXCD == 0) ? (b % NXCD) * (nb / NXCD) + b / NXCD : b;
+__device__ __forceinline__ PlaneGroup plane_group(int64_t nx, int64_t ny, int64_t bpp, int tyv) {
+    const int64_t t = blockIdx.x;
     const int64_t G = nx / VX;
     const int64_t g = (t % bpp) * (64 * tyv) + threadIdx.y * 64 + threadIdx.x;
     PlaneGroup r;
@@ -388,8 +381,7 @@ __device__ __forceinline__ void shell_pos(const ShellGeom &sg, uint32_t t, uint3
 // (the whole volume, one z-plane per workgroup) or shell entry t.  in_range is false for
 // lanes past the end; they still load from valid addresses and take part in the shuffles.
 struct LaneGroup { int64_t i0, j, k; bool in_range; };
-__device__ __forceinline__ LaneGroup lane_group(const ShellGeom &sg, int64_t nx, int64_t ny, int64_t bpp, int64_t nbz,
-                                                bool remap, int tyv) {
+__device__ __forceinline__ LaneGroup lane_group(const ShellGeom &sg, int64_t nx, int64_t ny, int64_t bpp, int tyv) {
     LaneGroup r;
     if (sg.n) {
         const uint32_t t = blockIdx.x * (uint32_t)(64 * tyv) + threadIdx.y * 64 + threadIdx.x;
@@ -400,7 +392,7 @@ __device__ __forceinline__ LaneGroup lane_group(const ShellGeom &sg, int64_t nx,
         r.j = (int64_t)j;
         r.k = (int64_t)k;
     } else {
-        const PlaneGroup pg = plane_group(nx, ny, bpp, nbz, remap, tyv);
+        const PlaneGroup pg = plane_group(nx, ny, bpp, tyv);
         r.i0 = pg.i0;
         r.j = pg.j;
         r.k = pg.bz;
@@ -425,10 +417,9 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
     const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
-    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, int64_t nbz, int remap,
-    ShellGeom sg) {
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, ShellGeom sg) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
-    const LaneGroup lg = lane_group(sg, nx, ny, nbx, nbz, remap != 0, TYV);
+    const LaneGroup lg = lane_group(sg, nx, ny, nbx, TYV);
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
     const bool act = lg.in_range && lg.j < ny - 1 && lg.k < nz - 1;
     const int64_t k = act ? lg.k : 0;
@@ -530,8 +521,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
     const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
     const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
-    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, int64_t nby,
-    int64_t nbz, int remap, ShellGeom sg) {
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, int64_t kchunk, float dt, int64_t nbx, ShellGeom sg) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
     __shared__ float4 tab[MI ? 256 : 1];
     if (MI) {
@@ -543,14 +533,14 @@ __global__ __launch_bounds__(64 * TYV) void yee_E_vec_kernel(
     bool act;
     if (LIST) {
         // one listed group per lane, one plane; every lane stays for the shuffles
-        const LaneGroup lg = lane_group(sg, nx, ny, 1, 1, false, TYV);
+        const LaneGroup lg = lane_group(sg, nx, ny, 1, TYV);
         act = lg.in_range && lg.j >= 1 && lg.k >= 1;
         i0 = lg.i0;
         j = lg.j;
         k0 = act ? lg.k : 1;
         k1 = k0 + 1;
     } else {
-        const PlaneGroup pg = plane_group(nx, ny, nbx, nbz, remap != 0, TYV);
+        const PlaneGroup pg = plane_group(nx, ny, nbx, TYV);
         i0 = pg.i0;
         j = pg.j;
         act = pg.in_plane && j >= 1;
@@ -869,22 +859,8 @@ static int check_yee(const mp_yee *g, const mp_cpml *c, const char *who) {
     return MP_OK;
 }
 
-// Measured on the 512x512x256 step the remap costs ~3% (the two x tiles of a row then sit
-// on different XCDs), so it is opt-in (MP_YEE_REMAP=1).
-static int xcd_remap() {
-    static const int on = getenv("MP_YEE_REMAP") != nullptr;
-    return on;
-}
-
-// Rows of 64 lanes per workgroup (MP_YEE_TYV: 2, 4 or 8; default 4).
-static int yee_tyv() {
-    static const int t = [] {
-        const char *e = getenv("MP_YEE_TYV");
-        const int v = e ? atoi(e) : 4;
-        return (v == 2 || v == 8) ? v : 4;
-    }();
-    return t;
-}
+// Rows of 64 lanes per workgroup of the vector kernels (2 and 8 measured no faster).
+constexpr int TYV = 4;
 
 // Interior box (groups gi0 .. gi0+ngi-1 of rows j0 .. j0+nj-1 of planes k0 .. k0+nk-1): the
 // float4 groups outside every CPML slab and off every boundary cell (a margin of at least one
@@ -910,11 +886,10 @@ static Box interior_box(const mp_yee *g, const mp_cpml *c) {
 }
 
 // The H and E updates split into interior and shell launches when the box is non-empty and
-// every group index, lanes past the end included, fits 32 bits (MP_YEE_NO_SPLIT=1 keeps one
-// launch over the whole volume).
+// every group index, lanes past the end included, fits 32 bits; kernel_form
+// MP_YEE_WHOLE keeps one launch over the whole volume.
 static bool split_ok(const mp_yee *g, const Box &b) {
-    static const bool off = getenv("MP_YEE_NO_SPLIT") != nullptr;
-    return !off && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32) - 4096;
+    return g->kernel_form != MP_YEE_WHOLE && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32) - 4096;
 }
 
 // Shell layout of a box (see ShellGeom); n = 0 when a divisor would be below 2, where the
@@ -962,8 +937,7 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         yee_H_vec_kernel<P, TY><<<(unsigned)(nbx * nbz), block, 0, stream()>>>(
             (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
             (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, nbz, xcd_remap(),
-            ShellGeom{});
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, ShellGeom{});
         MP_LAUNCH_CHECK("yee_H_vec_kernel");
         return MP_OK;
     }
@@ -978,7 +952,7 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
     yee_H_vec_kernel<P, TY><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
         (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, 1, 0, sg);
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sg);
     MP_LAUNCH_CHECK("yee_H_vec_kernel");
     return MP_OK;
 }
@@ -987,8 +961,7 @@ template <typename P, int TY>
 static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:103-152)
-    const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256 &&
-                    getenv("MP_YEE_NO_EPS_INDEX") == nullptr;
+    const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256;
     const Box b = interior_box(g, c);
     const ShellGeom sg = split_ok(g, b) ? shell_geom(g, b) : ShellGeom{};
     dim3 block(64, TY, 1);
@@ -1005,10 +978,10 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
         KernelTimer t("yee_E");
         if (mi)
             yee_E_vec_kernel<P, TY, true, false><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), ShellGeom{});
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, ShellGeom{});
         else
             yee_E_vec_kernel<P, TY, false, false><<<grid, block, 0, stream()>>>(
-                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, 1, nbz, xcd_remap(), ShellGeom{});
+                E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, nbx, ShellGeom{});
         MP_LAUNCH_CHECK("yee_E_vec_kernel");
         return MP_OK;
     }
@@ -1023,10 +996,10 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     const unsigned grid = cdiv(sg.n, 64 * TY);
     if (mi)
         yee_E_vec_kernel<P, TY, true, true><<<grid, block, 0, stream()>>>(
-            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, sg);
+            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, sg);
     else
         yee_E_vec_kernel<P, TY, false, true><<<grid, block, 0, stream()>>>(
-            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, 1, 1, 0, sg);
+            E_ARGS, make_cpml_args(c, false), nx, ny, nz, 1, (float)dt, 1, sg);
     MP_LAUNCH_CHECK("yee_E_vec_kernel");
 #undef INT_ARGS
 #undef E_ARGS
@@ -1035,20 +1008,12 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
 
 template <typename P>
 static int launch_H_vec(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
-    switch (yee_tyv()) {
-    case 2: return launch_H_vec_t<P, 2>(g, c, dt, mu0);
-    case 8: return launch_H_vec_t<P, 8>(g, c, dt, mu0);
-    default: return launch_H_vec_t<P, 4>(g, c, dt, mu0);
-    }
+    return launch_H_vec_t<P, TYV>(g, c, dt, mu0);
 }
 
 template <typename P>
 static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
-    switch (yee_tyv()) {
-    case 2: return launch_E_vec_t<P, 2>(g, c, dt);
-    case 8: return launch_E_vec_t<P, 8>(g, c, dt);
-    default: return launch_E_vec_t<P, 4>(g, c, dt);
-    }
+    return launch_E_vec_t<P, TYV>(g, c, dt);
 }
 
 // The vector form needs x rows of a multiple of 4 cells, the volumes and coefficient lines it
@@ -1056,8 +1021,7 @@ static int launch_E_vec(const mp_yee *g, const mp_cpml *c, double dt) {
 // of their elements; anything else takes the scalar kernels.
 static bool al(const void *p, size_t a) { return (uintptr_t)p % a == 0; }
 static bool vec_ok(const mp_yee *g, const mp_cpml *c, bool h) {
-    static const bool off = getenv("MP_YEE_SCALAR") != nullptr;
-    if (off || g->nx % VX != 0) return false;
+    if (g->kernel_form == MP_YEE_SCALAR || g->nx % VX != 0) return false;
     const void *v16[] = {g->ex, g->ey, g->ez, g->hx, g->hy, g->hz, g->dx, g->dy, g->dz,
                          g->inv_eps_x, g->inv_eps_y, g->inv_eps_z, g->inv_d_cell_x, g->inv_d_dual_x,
                          c->inv_kappa[0]};

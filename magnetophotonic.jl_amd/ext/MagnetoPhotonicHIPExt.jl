@@ -117,7 +117,7 @@ struct MpYee
     inv_eps_x::Ptr{Cvoid}; inv_eps_y::Ptr{Cvoid}; inv_eps_z::Ptr{Cvoid}
     inv_d_cell_x::Ptr{Cvoid}; inv_d_cell_y::Ptr{Cvoid}; inv_d_cell_z::Ptr{Cvoid}
     inv_d_dual_x::Ptr{Cvoid}; inv_d_dual_y::Ptr{Cvoid}; inv_d_dual_z::Ptr{Cvoid}
-    eps_index::Ptr{UInt8}; eps_table::Ptr{Cfloat}; n_eps::Int32; _pad_eps::Int32
+    eps_index::Ptr{UInt8}; eps_table::Ptr{Cfloat}; n_eps::Int32; kernel_form::Int32
 end
 
 struct MpCpml

@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 
 MP_F32, MP_F64, MP_BF16, MP_F16 = 0, 1, 2, 3
+MP_YEE_AUTO, MP_YEE_WHOLE, MP_YEE_SCALAR = 0, 1, 2
 MP_MAX_PLANES = 8
 
 vp = C.c_void_p
@@ -23,7 +24,7 @@ class Yee(C.Structure):
                 ("inv_eps_x", vp), ("inv_eps_y", vp), ("inv_eps_z", vp),
                 ("inv_d_cell_x", vp), ("inv_d_cell_y", vp), ("inv_d_cell_z", vp),
                 ("inv_d_dual_x", vp), ("inv_d_dual_y", vp), ("inv_d_dual_z", vp),
-                ("eps_index", vp), ("eps_table", vp), ("n_eps", i32), ("_pad_eps", i32)]
+                ("eps_index", vp), ("eps_table", vp), ("n_eps", i32), ("kernel_form", i32)]
 
 
 class Cpml(C.Structure):
@@ -91,7 +92,7 @@ class State(C.Structure):
                 ("multiphysics_every", i32), ("subcycles", i32), ("brillouin_iters", i32),
                 ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32),
                 ("surrogate", C.POINTER(Mlp)), ("surrogate_tol", f64), ("surrogate_fallbacks", vp),
-                ("film_lists_coincide", i32), ("_pad_state", i32)]
+                ("film_lists_coincide", i32), ("film_serial", i32)]
 
 
 MP_PULSE_GAUSSIAN, MP_PULSE_CARRIER_GAUSSIAN, MP_PULSE_CONTINUOUS = 1, 2, 3

@@ -64,7 +64,12 @@ def plane_wave_gain(dt: float, dz: float) -> float:
     return peak
 
 
-def small_coupled(dtype: str = "f32", steps: int = 2000, amplitude: float = 2.0e8, n: int = 64,
+# Soft-source amplitude of the 64^3 case: the film's electrons peak near 1.7e3 K over the
+# 2000 steps (1.49e3 K at 6.5e8 in the FP32 oracle).
+SMALL_COUPLED_AMPLITUDE = 7.5e8
+
+
+def small_coupled(dtype: str = "f32", steps: int = 2000, amplitude: float = SMALL_COUPLED_AMPLITUDE, n: int = 64,
                   absorption_model: str = "ade_work", source_kind: str = "plane"):
     nx = ny = nz = n
     d = 20e-9

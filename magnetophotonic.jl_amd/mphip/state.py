@@ -77,13 +77,25 @@ FIELD_NAMES = ("ex", "ey", "ez", "hx", "hy", "hz", "dx", "dy", "dz")
 FILM_NAMES = ("Te", "Tl", "Ts_TM", "Ts_RE", "m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z")
 
 
-class FDTDState:
-    """Device state of one coupled 3-D simulation; `step()` advances one EM step."""
+YEE_FORMS = {"auto": abi.MP_YEE_AUTO, "whole": abi.MP_YEE_WHOLE, "scalar": abi.MP_YEE_SCALAR}
 
-    def __init__(self, spec: StateSpec, device: int = 0):
+
+class FDTDState:
+    """Device state of one coupled 3-D simulation; `step()` advances one EM step.
+
+    use_eps_index: give the FP32 E update the 1-byte material index where the volumes hold
+    at most 256 distinct triples (False: it reads the three inv_eps volumes, as the Julia
+    extension's launcher does).  yee_form: "auto", "whole" or "scalar" kernel family of
+    the FP32 Yee updates.  film_serial: run the film update in step order on the launch
+    stream instead of beside the next step's Yee updates.  All of these give the same
+    values."""
+
+    def __init__(self, spec: StateSpec, device: int = 0, use_eps_index: bool = True, yee_form: str = "auto",
+                 film_serial: bool = False):
         _lib.init(device)
         self.spec = spec
         self._st = abi.State()
+        self._film_serial = film_serial
         self._dft = None
         self.n = 0
         self.t = 0.0
@@ -119,7 +131,8 @@ class FDTDState:
         y.inv_eps_x, y.inv_eps_y, y.inv_eps_z = (b.ptr for b in self.inv_eps)
         y.inv_d_cell_x, y.inv_d_cell_y, y.inv_d_cell_z = (b.ptr for b in self.idc)
         y.inv_d_dual_x, y.inv_d_dual_y, y.inv_d_dual_z = (b.ptr for b in self.idd)
-        ei = eps_index(spec.inv_eps) if FT == np.float32 and CT == np.float32 else None
+        y.kernel_form = YEE_FORMS[yee_form]
+        ei = eps_index(spec.inv_eps) if use_eps_index and FT == np.float32 and CT == np.float32 else None
         if ei is not None:
             self._eps_index, self._eps_table = dev_arr(ei[0]), dev_arr(ei[1])
             y.eps_index, y.eps_table, y.n_eps = self._eps_index.ptr, self._eps_table.ptr, ei[1].shape[0]
@@ -338,6 +351,7 @@ class FDTDState:
         st.brillouin_iters = spec.brillouin_iters
         st.has_ade_work = 1 if spec.has_ade_work() else 0
         st.film_lists_coincide = 1 if self._lists_coincide() else 0
+        st.film_serial = 1 if self._film_serial else 0
 
     def _lists_coincide(self) -> bool:
         """Every ADE list and the MO list is the film list (entry n is film cell n) and the

@@ -1,0 +1,132 @@
+"""Every selectable form of the step computes the same values.
+
+- The FP32 Yee updates: the interior + CPML-shell launches, one whole-volume vector launch
+  and the scalar kernels, each with the 1-byte material index and with the three inv_eps
+  volumes (the form the Julia extension's launcher selects, eps_index unset), bit for bit
+  against the oracle on a split shape and on a shape too small for the split.
+- The film update in step order on the launch stream (film_serial) against the default,
+  which runs it beside the next step's Yee updates on a side stream.
+- The C0 configuration as specified: 64^3, CPML 10, 2000 steps, fields bit for bit
+  against the FP32 oracle and the film magnetisation within 1e-4; the error of the same
+  run against the FP64 oracle is printed and held to a measured envelope.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("ex", "ey", "ez", "hx", "hy", "hz", "dx", "dy", "dz")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _case_spec(case):
+    from mphip import configs
+    from test_gpu_parity import _em_box_spec
+    if case == "pump_96x68x64":      # interior box + CPML shell where the form allows
+        return configs.coupled_pump(nx=96, ny=68, nz=64, steps=400, fluence_mj_cm2=12.0, npml=6, src_z=40e-9)[0]
+    return _em_box_spec(36, 20, 22, npml=3, seed=56)   # an empty interior box: one launch
+
+
+@pytest.mark.parametrize("form", ["auto", "whole", "scalar"])
+@pytest.mark.parametrize("eps_index", [True, False])
+@pytest.mark.parametrize("case", ["pump_96x68x64", "boxes_36x20x22"])
+def test_yee_forms_bitwise(form, eps_index, case):
+    import driver
+    from mphip.state import FDTDState
+    from test_gpu_fullsize import _seed_fields
+    spec = _case_spec(case)
+    print(f"case yee form={form} eps_index={eps_index} {case}", flush=True)
+    gpu = FDTDState(spec, use_eps_index=eps_index, yee_form=form)
+    assert (gpu.yee.n_eps > 0) == eps_index
+    ref = driver.OracleState(spec)
+    _seed_fields(gpu, ref, spec.shape, 11)
+    gpu.run(4)
+    ref.run(4)
+    gpu.synchronize()
+    for name in FIELDS:
+        a, b = gpu.field(name), ref.field(name)
+        assert np.array_equal(a, b), (name, int(np.count_nonzero(a != b)))
+    for s in range(12):
+        assert np.array_equal(gpu.psi("h", s), ref.psi_h[s]), ("psi_h", s)
+        assert np.array_equal(gpu.psi("d", s), ref.psi_d[s]), ("psi_d", s)
+    gpu.free()
+
+
+def test_film_serial_matches_side_stream():
+    from mphip import configs
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+    spec, meta = configs.coupled_pump(nx=44, ny=40, nz=64, steps=240, fluence_mj_cm2=14.0, npml=6, src_z=40e-9)
+    print("case film_serial vs side stream 44x40x64, 150 steps", flush=True)
+    a, b = FDTDState(spec), FDTDState(spec, film_serial=True)
+    ma, mb = DFTPlanes(a, meta["planes"], meta["omega_bins"]), DFTPlanes(b, meta["planes"], meta["omega_bins"])
+    a.attach_dft(ma)
+    b.attach_dft(mb)
+    a.run(150)
+    b.run(75)
+    b.run(75)
+    a.synchronize()
+    b.synchronize()
+    for name in FIELDS:
+        assert np.array_equal(a.field(name), b.field(name)), name
+    fa, fb = a.film_state(), b.film_state()
+    assert fb["Te"].max() > 300.5
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+    for p in range(len(meta["planes"])):
+        ga, gb = ma.download(p), mb.download(p)
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), (p, k)
+    a.free()
+    b.free()
+
+
+@pytest.mark.timeout(600)
+def test_c0_small_coupled_2000_steps():
+    """64^3 uniform 20 nm, CPML 10, Si3N4 slab under a 2-cell Drude + Lorentz film with
+    4TM + LLB from 300 K and m = +-0.9, Ex pulse at 800 nm / 50 fs FWHM, 2000 steps."""
+    import driver
+    from mphip import configs
+    from mphip.state import FDTDState
+    spec, meta = configs.small_coupled(dtype="f32", steps=2000)
+    assert spec.shape == (64, 64, 64) and spec.cpml.widths == (10, 10, 10)
+    print(f"case C0 {meta['name']} 2000 steps", flush=True)
+    gpu = FDTDState(spec)
+    ref = driver.OracleState(spec)
+    te_peak = 0.0
+    for _ in range(20):
+        gpu.run(100)
+        ref.run(100)
+        te_peak = max(te_peak, float(ref.film_state()["Te"].max()))
+    gpu.synchronize()
+    for name in FIELDS:
+        a, b = gpu.field(name), ref.field(name)
+        assert np.array_equal(a, b), (name, int(np.count_nonzero(a != b)))
+    fa, fb = gpu.film_state(), ref.film_state()
+    assert te_peak > 1200.0, te_peak
+    for k in ("Te", "Tl", "Ts_TM", "Ts_RE"):
+        assert _rel(fa[k], fb[k]) < 1e-6, k
+    for k in ("m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z"):
+        assert np.abs(fa[k].astype(np.float64) - fb[k]).max() < 1e-4, k
+    # the same run against the FP64 oracle (fields and film state in FP64 throughout)
+    spec64, _ = configs.small_coupled(dtype="f64", steps=2000)
+    r64 = driver.OracleState(spec64)
+    r64.run(2000)
+    f64 = r64.film_state()
+    err = {n: _rel(gpu.field(n), r64.field(n)) for n in ("ex", "ey", "ez", "hx", "hy", "hz")}
+    err["Te"] = _rel(fa["Te"], f64["Te"])
+    err["m_TM_x"] = float(np.abs(fa["m_TM_x"].astype(np.float64) - f64["m_TM_x"]).max())
+    err["m_RE_x"] = float(np.abs(fa["m_RE_x"].astype(np.float64) - f64["m_RE_x"]).max())
+    print("C0 error against the FP64 oracle:", {k: f"{v:.2e}" for k, v in err.items()}, flush=True)
+    # measured in the FP32 oracle (bitwise the GPU's fields): Ex 2.1e-4, Ey 3.6e-2 (the
+    # cross-polarised field, made only by the gyration and orders below Ex), Ez 1.3e-5,
+    # Hx 2.0e-4, Hy 3.3e-6, Hz 4.8e-5; Te 1.2e-5, m_TM_x 5.9e-5, m_RE_x 5.2e-6
+    for n in ("ex", "ez", "hx", "hy", "hz"):
+        assert err[n] < 1e-3, (n, err[n])
+    assert err["ey"] < 0.1, err["ey"]
+    assert err["Te"] < 5e-5 and err["m_TM_x"] < 1e-4 and err["m_RE_x"] < 1e-4, err
+    gpu.free()
