@@ -440,6 +440,18 @@ int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, 
                          int32_t nb, int32_t center_bin, const double *mode_w, const double *area,
                          double *out);
 
+/* mp_probe_shot_reduce without any allocation: the same kernels, partition and output,
+ * with the partial sums in a fixed device array that the library holds (one per GPU,
+ * 320 KiB, kept until the process ends) instead of a buffer it allocates and regrows.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  nb must not exceed
+ * MP_SHOT_NOALLOC_MAX_BINS; larger nb is rejected with MP_ERR_ARG.  Calls on one GPU
+ * share the array: calls that overlap on one GPU (several streams or host threads) race. */
+#define MP_SHOT_NOALLOC_MAX_BINS 63
+int mp_probe_shot_reduce_noalloc(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2,
+                                 int32_t nb, int32_t center_bin, const double *mode_w,
+                                 const double *area, double *out);
+
 /* Weighted plane sum of one field component (0..5 = Ex, Ey, Ez, Hx, Hy, Hz) on the plane
  * `index` normal to `axis`: out[0] = sum(F * weight) over the plane cells in (n1, n2)
  * order, weight = 1 when null (device, n1*n2 doubles).  Replaces the host loops of

@@ -721,21 +721,24 @@ static int shot_plane(const mp_plane *p, ShotPlane &s) {
 
 }  // namespace mp
 
-MP_API int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2, int32_t nb,
-                                int32_t center_bin, const double *mode_w, const double *area, double *out) {
-    if (!pre || !post || !mode_w || !area || !out) return fail_arg("mp_probe_shot_reduce: null argument");
-    if (n1 < 1 || n2 < 1) return fail_arg("mp_probe_shot_reduce: plane sizes must be positive");
-    if (nb < 1 || nb > 65535) return fail_arg("mp_probe_shot_reduce: nb must be in [1, 65535]");
-    if (center_bin < 0 || center_bin >= nb) return fail_arg("mp_probe_shot_reduce: center_bin out of range");
-    const int64_t np = n1 * n2;
-    ShotPlane a, b;
-    int rc = shot_plane(pre, a);
-    if (rc) return rc;
-    rc = shot_plane(post, b);
-    if (rc) return rc;
-    double *part;
-    rc = shot_partials(nb, &part);
-    if (rc) return rc;
+namespace mp {
+
+// Argument checks shared by the two shot-reduction entry points.
+static int shot_args(const char *who, const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2, int32_t nb,
+                     int32_t center_bin, const double *mode_w, const double *area, const double *out, ShotPlane &a,
+                     ShotPlane &b) {
+    if (!pre || !post || !mode_w || !area || !out) return fail_arg("%s: null argument", who);
+    if (n1 < 1 || n2 < 1) return fail_arg("%s: plane sizes must be positive", who);
+    if (nb < 1 || nb > 65535) return fail_arg("%s: nb must be in [1, 65535]", who);
+    if (center_bin < 0 || center_bin >= nb) return fail_arg("%s: center_bin out of range", who);
+    if (int rc = shot_plane(pre, a)) return rc;
+    return shot_plane(post, b);
+}
+
+// The two launches of a shot reduction; part holds SHOT_NV * SHOT_BLOCKS * (nb + 1)
+// doubles, every one of which the first launch writes before the second reads it.
+static int shot_launch(const ShotPlane &a, const ShotPlane &b, int64_t np, int32_t nb, int32_t center_bin,
+                       const double *mode_w, const double *area, double *part, double *out) {
     KernelTimer t("probe_shot_reduce");
     shot_partial_kernel<<<dim3(SHOT_BLOCKS, nb + 1), RED_THREADS, 0, stream()>>>(a, b, mode_w, area, np, nb,
                                                                                    center_bin, part);
@@ -744,6 +747,60 @@ MP_API int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64
     shot_finalize_kernel<<<(unsigned)cdiv(nout, 256), 256, 0, stream()>>>(part, nb, out);
     MP_LAUNCH_CHECK("shot_finalize_kernel");
     return MP_OK;
+}
+
+// Partials of mp_probe_shot_reduce_noalloc: a fixed device array, one copy per GPU, held
+// until the process ends.  Host code never uses the name `scratch` as an address (in host
+// code it is a stand-in, not device memory): it uses the address hipGetSymbolAddress gives
+// for the current device, resolved once per device at mp_init or at the first call.
+constexpr int SHOT_NOALLOC_DOUBLES = SHOT_NV * SHOT_BLOCKS * (MP_SHOT_NOALLOC_MAX_BINS + 1);
+static __device__ __align__(256) double scratch[SHOT_NOALLOC_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+static double *g_scratch_dev[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's copy of `scratch` (resolved with that device current).
+int noalloc_scratch(double **out) {
+    int dev = 0;
+    MP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail_arg("no-allocation scratch: device %d beyond %d", dev, MAX_DEVICES);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        MP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail_arg("no-allocation scratch: no device address on device %d", dev);
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return MP_OK;
+}
+
+}  // namespace mp
+
+MP_API int mp_probe_shot_reduce(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2, int32_t nb,
+                                int32_t center_bin, const double *mode_w, const double *area, double *out) {
+    ShotPlane a, b;
+    if (int rc = shot_args("mp_probe_shot_reduce", pre, post, n1, n2, nb, center_bin, mode_w, area, out, a, b))
+        return rc;
+    double *part;
+    if (int rc = shot_partials(nb, &part)) return rc;
+    return shot_launch(a, b, n1 * n2, nb, center_bin, mode_w, area, part, out);
+}
+
+MP_API int mp_probe_shot_reduce_noalloc(const mp_plane *pre, const mp_plane *post, int64_t n1, int64_t n2,
+                                        int32_t nb, int32_t center_bin, const double *mode_w, const double *area,
+                                        double *out) {
+    ShotPlane a, b;
+    if (int rc = shot_args("mp_probe_shot_reduce_noalloc", pre, post, n1, n2, nb, center_bin, mode_w, area, out, a,
+                           b))
+        return rc;
+    if (nb > MP_SHOT_NOALLOC_MAX_BINS)
+        return fail_arg("mp_probe_shot_reduce_noalloc: nb = %d exceeds the maximum of %d bins of the fixed scratch",
+                        nb, MP_SHOT_NOALLOC_MAX_BINS);
+    double *part;
+    if (int rc = noalloc_scratch(&part)) return rc;
+    return shot_launch(a, b, n1 * n2, nb, center_bin, mode_w, area, part, out);
 }
 
 MP_API int mp_plane_overlap(const mp_yee *g, int32_t component, int32_t axis, int64_t index, const double *weight,

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.h"
+#include "monitors.h"
 
 namespace mp {
 
@@ -113,7 +114,8 @@ MP_API int mp_init(int device) {
     if (device < 0 || device >= n) return fail_arg("mp_init: device %d out of range (%d devices)", device, n);
     MP_HIP(hipSetDevice(device));
     MP_HIP(hipFree(nullptr));  // create the context now
-    return MP_OK;
+    double *scratch_dev = nullptr;
+    return noalloc_scratch(&scratch_dev);  // this device's fixed scratch, resolved once
 }
 
 MP_API const char *mp_last_error(void) { return g_err; }

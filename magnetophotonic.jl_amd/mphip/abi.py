@@ -122,6 +122,12 @@ SIGNATURES = {
     "last_error": (C.c_char_p, []),
 }
 
+# No-allocation variants of entry points (fixed library scratch); no oracle counterpart.
+NOALLOC_SIGNATURES = {
+    "probe_shot_reduce_noalloc": (C.c_int, [P(Plane), P(Plane), i64, i64, i32, i32, vp, vp, vp]),
+}
+MP_SHOT_NOALLOC_MAX_BINS = 63
+
 DRIVER_SIGNATURES = {
     "run_steps": (C.c_int, [P(State), i64]),
     "pulse_value": (C.c_double, [P(Pulse), f64]),
