@@ -130,3 +130,33 @@ def test_c0_small_coupled_2000_steps():
     assert err["ey"] < 0.1, err["ey"]
     assert err["Te"] < 5e-5 and err["m_TM_x"] < 1e-4 and err["m_RE_x"] < 1e-4, err
     gpu.free()
+
+
+@pytest.mark.parametrize("axis,component", [("x", "Ey"), ("y", "Ez"), ("z", "Ex"), ("x", "Hz")])
+@pytest.mark.parametrize("frac", [0.0, 0.37])
+def test_plane_source_every_axis_bitwise(axis, component, frac):
+    """The plane source on each axis, on a cell centre (one weight) and between two centres
+    (two weights summing to 1), against the oracle bit for bit (random dielectric boxes,
+    CPML 3, interior/shell split)."""
+    import math
+    import driver
+    from mphip import grid as G, sources
+    from mphip.state import FDTDState
+    from test_gpu_parity import _em_box_spec
+    spec = _em_box_spec(96, 68, 64, npml=3, seed=17)
+    ax = {"x": spec.grid.x, "y": spec.grid.y, "z": spec.grid.z}[axis]
+    c = ax.centers
+    k = len(c) // 3
+    pos = c[k] + frac * (c[k + 1] - c[k])
+    pulse = sources.GaussianPulse(amplitude=1e3, t0=0.5e-15, tau=0.2e-15, omega=2 * math.pi * G.C0 / 800e-9)
+    spec.source = sources.PlaneSource(pulse, component, axis=axis, position=pos)
+    print(f"case plane source axis={axis} {component} frac={frac}", flush=True)
+    gpu, ref = FDTDState(spec), driver.OracleState(spec)
+    gpu.run(60)
+    ref.run(60)
+    gpu.synchronize()
+    for name in FIELDS:
+        a, b = gpu.field(name), ref.field(name)
+        assert np.array_equal(a, b), (name, int(np.count_nonzero(a != b)))
+    assert np.abs(ref.field(component.lower())).max() > 0
+    gpu.free()
