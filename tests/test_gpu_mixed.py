@@ -9,7 +9,7 @@ Accuracy: the coupled pump on a 128 x 128 x 64 copy of the pump geometry (60 mJ/
 steps, then 600 relax steps of 5 fs) switches about 5 % of the film.  Each storage form is
 compared with the FP64 oracle on identical inputs.  Plain FP32 meets the film bar (mean
 |dm| <= 1e-3); binary16 storage stays within a few per cent on fields and temperature but
-moves the switched fraction; BF16 storage is outside both (measured, not asserted here).
+moves the switched fraction; BF16 storage is far outside both (held to its measured envelope).
 """
 import numpy as np
 import pytest
@@ -136,7 +136,7 @@ def errors(psi, ade):
 
 
 @pytest.mark.timeout(900)
-@pytest.mark.parametrize("kind", ["f32", "f16"])
+@pytest.mark.parametrize("kind", ["f32", "f16", "bf16"])
 def test_storage_form_vs_fp64_oracle(kind):
     print(f"case storage form {kind} vs FP64 oracle 128x128x64", flush=True)
     e = errors(kind, kind)
@@ -146,9 +146,17 @@ def test_storage_form_vs_fp64_oracle(kind):
         assert e["mean_dm_m_TM_x"] <= 1e-3 and e["mean_dm_m_RE_x"] <= 1e-3
         assert e["Te_peak_rel"] < 1e-3
         assert e["relL2_E"] < 1e-4 and e["relL2_H"] < 1e-4
-    else:
+    elif kind == "f16":
         # binary16 psi + ADE state: measured envelope (fields ~1 %, Te ~5 %, switched
         # fraction within 0.5 % of the film); the film-mean bar of 1e-3 is not met
         assert e["relL2_E"] < 5e-2 and e["relL2_H"] < 2e-2
         assert e["Te_peak_rel"] < 0.1
         assert abs(e["switched"] - e["switched_ref"]) < 5e-3
+    else:
+        # BF16 psi + ADE state (the form of the 512 x 512 x 256 mixed-storage workload): the
+        # 8-bit mantissa of P and J drives the ADE recursion off (measured in the oracle's
+        # restatement of this storage: E 0.66, H 0.13, Te peak +123 %, film-mean |dm| 1.23,
+        # 68.5 % switched against 6.3 %); held to that envelope, far outside the film bar
+        assert e["relL2_E"] < 1.0 and e["relL2_H"] < 0.3
+        assert e["Te_peak_rel"] < 2.0
+        assert e["mean_dm_m_TM_x"] > 1e-3 and e["switched"] > 0.3
