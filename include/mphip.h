@@ -290,10 +290,10 @@ typedef struct mp_state {
 int mp_init(int device);
 /* Thread-local message of the last failing call ("" if none). */
 const char *mp_last_error(void);
-/* zeros_backend / free_device! (Backend.jl:37-48,121-126). */
+/* zeros_backend / free_device! (Backend.jl:31-33,126). */
 int mp_malloc(void **ptr, size_t bytes);
 int mp_free(void *ptr);
-/* adapt_backend / to_host (Backend.jl:49-60); synchronous copies. */
+/* adapt_backend / to_host (Backend.jl:41-47); synchronous copies. */
 int mp_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int mp_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int mp_memcpy_d2d(void *dst, const void *src, size_t bytes);
@@ -521,7 +521,7 @@ int mp_film_reduce(const mp_film *f, double *out);
 
 /* The sums behind one FilmAverage / SwitchedFraction record (src/sim/Monitors.jl:
  * 518-570, 487-497; reduce_mean / reduce_norm_mean / reduce_count_negative,
- * src/core/Backend.jl:48-57), over the f->n film cells in FP64: out[0..5] = sum of
+ * src/core/Backend.jl:49-57), over the f->n film cells in FP64: out[0..5] = sum of
  * m_TM_x, m_TM_y, m_TM_z, m_RE_x, m_RE_y, m_RE_z; out[6], out[7] = sum of |m_TM|, |m_RE|;
  * out[8..11] = sum of Te, Tl, Ts_TM, Ts_RE; out[12] = count of m_TM_x < 0.  The caller
  * divides by n.  out: 13 device doubles (zeros when n == 0).  Fixed reduction order. */
@@ -537,8 +537,37 @@ int mp_film_average(const mp_film *f, double *out);
 int mp_switching_metrics(const mp_film *f, double core_fraction, double *out);
 
 /* Non-finite scan of a device array (NaNGuard, Monitors.jl:580-606): *flag (device
- * int32) is set to 1 if any element is NaN or Inf; the caller clears it. */
+ * int32) is set to 1 if any element is NaN or Inf; the caller clears it.  Replaces
+ * any_nonfinite (src/core/Backend.jl:59). */
 int mp_any_nonfinite(const void *x, int32_t dtype, int64_t n, int32_t *flag);
+
+/* One FP64 reduction of a device array of n elements (dtype MP_F32 or MP_F64) into
+ * out[0] (device): MP_RED_SUM sum(x); MP_RED_COUNT_NEGATIVE the count of x < 0;
+ * MP_RED_NORM_SUM sum(sqrt(x^2 + y^2 + z^2)) over three arrays.  0 when n == 0.  Fixed
+ * partition and reduction order.  Replaces reduce_sum, reduce_mean (the caller divides
+ * by n), reduce_count_negative and reduce_norm_mean (src/core/Backend.jl:49-57), which
+ * the reference evaluates with GPUArrays mapreduce on device arrays. */
+enum mp_reduce_op { MP_RED_SUM = 0, MP_RED_COUNT_NEGATIVE = 1, MP_RED_NORM_SUM = 2 };
+int mp_reduce(const void *x, const void *y, const void *z, int32_t dtype, int64_t n, int32_t op,
+              double *out);
+
+/* out[b] = sum of x[b*block .. (b+1)*block - 1] for b < nblocks (FP64, device arrays):
+ * sum(work_trace; dims=(1, 2)) of the probe trace buffer (src/sim/ProbeReadout.jl:291). */
+int mp_sum_blocks(const double *x, int64_t block, int64_t nblocks, double *out);
+
+/* plane_to_host's device side (src/core/Backend.jl:66-83): the plane `index` (0-based)
+ * normal to `axis` of a (nx, ny, nz) field copied to out (device) in the order of the
+ * reference's views: x -> field[i,:,:] as (ny, nz), y -> field[:,i,:] as (nx, nz),
+ * z -> field[:,:,i] as (nx, ny), first index fastest. */
+int mp_plane_gather(const void *field, int32_t dtype, int64_t nx, int64_t ny, int64_t nz,
+                    int32_t axis, int64_t index, void *out);
+
+/* probe_trace_plane_kernel! (src/fdtd/Kernels.jl:467-478): work (ny, nz, 4), device FP64,
+ * = Ey[i_pre]*w, Ez[i_pre]*w, Ey[i_post]*w, Ez[i_post]*w with w = mode_w*area (ny*nz
+ * doubles each) on the two x-normal planes.  Replaces _ka_probe_trace_plane!
+ * (Kernels.jl:691-699); mp_probe_trace_plane is the same with the plane sums fused. */
+int mp_probe_trace_samples(const mp_yee *g, int64_t i_pre, int64_t i_post, const double *mode_w,
+                           const double *area, double *work);
 
 #ifdef __cplusplus
 }

@@ -929,3 +929,146 @@ MP_API int mp_any_nonfinite(const void *x, int32_t dtype, int64_t n, int32_t *fl
     MP_LAUNCH_CHECK("nonfinite_kernel");
     return MP_OK;
 }
+
+// ---- generic device reductions and plane copies behind the backend hooks ---------------------
+// reduce_sum / reduce_mean / reduce_count_negative / reduce_norm_mean (src/core/Backend.jl:
+// 42-55), sum(x; dims) over leading dimensions (src/sim/ProbeReadout.jl:291) and
+// plane_to_host (Backend.jl:61-83) on device arrays, which the reference computes with
+// GPUArrays mapreduce and views.  Fixed partitions and ordered final sums: a call gives
+// the same result every time.
+namespace mp {
+
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void reduce_partial_kernel(const T *x, const T *y, const T *z, int64_t n,
+                                                                     int op, double *part) {
+    double v[1] = {0.0};
+    for (int64_t i = (int64_t)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (int64_t)RED_BLOCKS * RED_THREADS) {
+        const double a = (double)x[i];
+        if (op == MP_RED_SUM) {
+            v[0] += a;
+        } else if (op == MP_RED_COUNT_NEGATIVE) {
+            v[0] += x[i] < (T)0 ? 1.0 : 0.0;
+        } else {
+            const double b = (double)y[i], c = (double)z[i];
+            v[0] += sqrt(a * a + b * b + c * c);
+        }
+    }
+    block_sum_store<1>(v, part + blockIdx.x);
+}
+
+// out[b] = sum of block b (block elements, consecutive), one workgroup per block.
+__global__ __launch_bounds__(RED_THREADS) void sum_blocks_kernel(const double *x, int64_t block, double *out) {
+    double v[1] = {0.0};
+    const double *xb = x + (int64_t)blockIdx.x * block;
+    for (int64_t i = threadIdx.x; i < block; i += RED_THREADS) v[0] += xb[i];
+    block_sum_store<1>(v, out + blockIdx.x);
+}
+
+// The plane `index` normal to `axis` in the order of the reference's views: x -> field[i,:,:]
+// (ny, nz), y -> field[:,i,:] (nx, nz), z -> field[:,:,i] (nx, ny), first index fastest.
+template <typename T>
+__global__ __launch_bounds__(256) void plane_gather_kernel(const T *f, int axis, int64_t idx, int64_t nx, int64_t ny,
+                                                           int64_t n1, int64_t n2, T *out) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n1 * n2) return;
+    const int64_t a = c % n1, b = c / n1;
+    int64_t o;
+    if (axis == 0) o = idx + nx * (a + ny * b);
+    else if (axis == 1) o = a + nx * (idx + ny * b);
+    else o = a + nx * (b + ny * idx);
+    out[c] = f[o];
+}
+
+// probe_trace_plane_kernel! (Kernels.jl:467-478) on x-normal planes i_pre, i_post:
+// work (ny, nz, 4) = Ey_pre*w, Ez_pre*w, Ey_post*w, Ez_post*w with w = mode_w*area.
+template <typename S>
+__global__ __launch_bounds__(256) void trace_samples_kernel(const S *Ey, const S *Ez, int64_t ipre, int64_t ipost,
+                                                            const double *mode_w, const double *area, int64_t nx,
+                                                            int64_t ny, int64_t nz, double *work) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t np = ny * nz;
+    if (c >= np) return;
+    const double w = mode_w[c] * area[c];
+    const int64_t o1 = ipre + nx * c, o2 = ipost + nx * c;
+    work[c] = (double)Ey[o1] * w;
+    work[c + np] = (double)Ez[o1] * w;
+    work[c + 2 * np] = (double)Ey[o2] * w;
+    work[c + 3 * np] = (double)Ez[o2] * w;
+}
+
+}  // namespace mp
+
+MP_API int mp_reduce(const void *x, const void *y, const void *z, int32_t dtype, int64_t n, int32_t op, double *out) {
+    if (!out) return fail_arg("mp_reduce: null output");
+    if (op != MP_RED_SUM && op != MP_RED_COUNT_NEGATIVE && op != MP_RED_NORM_SUM)
+        return fail_arg("mp_reduce: unknown op %d", op);
+    if (n <= 0) return mp_fill(out, MP_F64, 1, 0.0);
+    if (!x || (op == MP_RED_NORM_SUM && (!y || !z))) return fail_arg("mp_reduce: null input");
+    double *part;
+    if (int rc = partials(&part)) return rc;
+    KernelTimer t("reduce");
+    if (dtype == MP_F32)
+        reduce_partial_kernel<float><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const float *)x, (const float *)y,
+                                                                              (const float *)z, n, op, part);
+    else if (dtype == MP_F64)
+        reduce_partial_kernel<double><<<RED_BLOCKS, RED_THREADS, 0, stream()>>>((const double *)x, (const double *)y,
+                                                                               (const double *)z, n, op, part);
+    else
+        return fail_arg("mp_reduce: unsupported dtype %d", dtype);
+    MP_LAUNCH_CHECK("reduce_partial_kernel");
+    finalize_kernel<<<1, 64, 0, stream()>>>(part, 1, 1, 1, 1.0, out);
+    MP_LAUNCH_CHECK("finalize_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_sum_blocks(const double *x, int64_t block, int64_t nblocks, double *out) {
+    if (!x || !out) return fail_arg("mp_sum_blocks: null argument");
+    if (block < 0 || nblocks < 0 || nblocks > 65535) return fail_arg("mp_sum_blocks: bad sizes");
+    if (nblocks == 0) return MP_OK;
+    if (block == 0) return mp_fill(out, MP_F64, nblocks, 0.0);
+    KernelTimer t("sum_blocks");
+    sum_blocks_kernel<<<(unsigned)nblocks, RED_THREADS, 0, stream()>>>(x, block, out);
+    MP_LAUNCH_CHECK("sum_blocks_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_plane_gather(const void *field, int32_t dtype, int64_t nx, int64_t ny, int64_t nz, int32_t axis,
+                           int64_t index, void *out) {
+    if (!field || !out) return fail_arg("mp_plane_gather: null argument");
+    if (nx < 1 || ny < 1 || nz < 1) return fail_arg("mp_plane_gather: bad dims");
+    if (axis < 0 || axis > 2) return fail_arg("3-D field plane axis must be :x, :y, or :z");
+    const int64_t nax = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    if (index < 0 || index >= nax) return fail_arg("mp_plane_gather: plane index out of range");
+    const int64_t n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+    KernelTimer t("plane_gather");
+    if (dtype == MP_F32)
+        plane_gather_kernel<float><<<cdiv(n1 * n2, 256), 256, 0, stream()>>>((const float *)field, axis, index, nx, ny,
+                                                                             n1, n2, (float *)out);
+    else if (dtype == MP_F64)
+        plane_gather_kernel<double><<<cdiv(n1 * n2, 256), 256, 0, stream()>>>((const double *)field, axis, index, nx,
+                                                                              ny, n1, n2, (double *)out);
+    else
+        return fail_arg("mp_plane_gather: unsupported dtype %d", dtype);
+    MP_LAUNCH_CHECK("plane_gather_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_probe_trace_samples(const mp_yee *g, int64_t i_pre, int64_t i_post, const double *mode_w,
+                                  const double *area, double *work) {
+    if (!g || !mode_w || !area || !work) return fail_arg("mp_probe_trace_samples: null argument");
+    if (i_pre < 0 || i_pre >= g->nx || i_post < 0 || i_post >= g->nx)
+        return fail_arg("mp_probe_trace_samples: plane index out of range");
+    const int64_t np = g->ny * g->nz;
+    if (np == 0) return MP_OK;
+    KernelTimer t("trace_samples");
+    if (g->field_dtype == MP_F32)
+        trace_samples_kernel<float><<<cdiv(np, 256), 256, 0, stream()>>>(
+            (const float *)g->ey, (const float *)g->ez, i_pre, i_post, mode_w, area, g->nx, g->ny, g->nz, work);
+    else if (g->field_dtype == MP_F64)
+        trace_samples_kernel<double><<<cdiv(np, 256), 256, 0, stream()>>>(
+            (const double *)g->ey, (const double *)g->ez, i_pre, i_post, mode_w, area, g->nx, g->ny, g->nz, work);
+    else
+        return fail_arg("mp_probe_trace_samples: unsupported field dtype");
+    MP_LAUNCH_CHECK("trace_samples_kernel");
+    return MP_OK;
+}

@@ -11,13 +11,19 @@ module MagnetoPhotonicHIPExt
 #
 # Device arrays are `HIPArray`s: a pointer from mp_malloc plus dims, freed by a
 # finalizer (or eagerly by free_device!).  Integer index vectors adapted to the device
-# become Int32 and 0-based (cell lists, position maps with 0 -> -1), the library's
-# convention.
+# become `HIPIndexArray`s: Int32 and 0-based on the device (cell lists, position maps with
+# 0 -> -1), the library's convention, and 1-based again when `to_host` downloads them.
+#
+# The step: `step!` reaches `_step_typed!`, whose `::HIPBackend` method runs the whole
+# step sequence as one `mp_run_steps` call (fused film pass, side-stream film update);
+# `run!(state, steps)` without a callback issues all the steps in one call.  The
+# per-launcher methods below serve callers that drive the launchers themselves.
 
 using MagnetoPhotonic
 import MagnetoPhotonic: HIPBackend, FieldState, Grid3D, FDTDParams, DLPole, ADEState,
                         MagnetoOpticADEState, ThermalState, MagnetizationState,
-                        AbsorptionState, MagnetoOpticModel, FerrimagnetParameters
+                        AbsorptionState, MagnetoOpticModel, FerrimagnetParameters, FDTDState,
+                        PlaneSource, ModeSource, GaussianPulse, CarrierGaussianPulse, ContinuousSource
 import Adapt
 
 const libmphip = get(ENV, "MPHIP_LIBRARY", joinpath(@__DIR__, "..", "mphip", "libmphip.so"))
@@ -79,14 +85,45 @@ function Base.fill!(a::HIPArray{T}, v) where {T<:Union{Float32,Float64}}
     return a
 end
 
+function Base.copyto!(dst::HIPArray{T}, src::HIPArray{T}) where {T}
+    length(dst) == length(src) || throw(DimensionMismatch("copyto! sizes differ"))
+    isempty(src) || _check(ccall((:mp_memcpy_d2d, libmphip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                                 dst.ptr, src.ptr, length(src) * sizeof(T)), "mp_memcpy_d2d")
+    return dst
+end
+
 function _upload(x::Array{T,N}) where {T,N}
     d = HIPArray{T,N}(size(x))
     isempty(x) || copyto!(d, x)
     return d
 end
 
-# Index lists: Int32, 0-based (0 = "absent" becomes -1).
-_upload_index(x::AbstractArray{<:Integer}) = _upload(Int32.(x) .- Int32(1))
+# Index arrays: Int32 and 0-based on the device (0 = "absent" becomes -1); to_host gives
+# the reference's 1-based Int values back.
+struct HIPIndexArray{N} <: AbstractArray{Int,N}
+    data::HIPArray{Int32,N}
+end
+Base.size(a::HIPIndexArray) = size(a.data)
+Base.IndexStyle(::Type{<:HIPIndexArray}) = IndexLinear()
+Base.getindex(::HIPIndexArray, ::Int) = error("scalar indexing of a HIPIndexArray; copy it with to_host first")
+_upload_index(x::AbstractArray{<:Integer}) = HIPIndexArray(_upload(Int32.(x) .- Int32(1)))
+
+# dest .= src (a HIPArray or a host Array of the same element type) and dest .= scalar:
+# the broadcasts the reference applies to device arrays (apply_magnetization!,
+# Solver.jl:474-483; fill!-style resets).  Anything else is refused.
+struct HIPStyle <: Broadcast.AbstractArrayStyle{Any} end
+HIPStyle(::Val) = HIPStyle()
+Base.BroadcastStyle(::Type{<:HIPArray}) = HIPStyle()
+Base.BroadcastStyle(::HIPStyle, ::Broadcast.DefaultArrayStyle) = HIPStyle()
+function Base.copyto!(dest::HIPArray{T}, bc::Broadcast.Broadcasted{HIPStyle}) where {T}
+    if bc.f === identity && length(bc.args) == 1
+        src = bc.args[1]
+        (src isa HIPArray{T} || src isa Array{T}) && return copyto!(dest, src)
+        src isa Number && return fill!(dest, src)
+    end
+    throw(ArgumentError("HIPArray broadcasting supports dest .= src (a HIPArray or Array of the same " *
+                        "element type) and dest .= scalar"))
+end
 
 # ---- backend hooks (src/core/Backend.jl) --------------------------------------------------------
 MagnetoPhotonic.zeros_backend(::HIPBackend, ::Type{T}, dims::Integer...) where {T} =
@@ -97,8 +134,10 @@ MagnetoPhotonic.adapt_backend(::HIPBackend, x::Array{<:DLPole}) = _upload(x)
 MagnetoPhotonic.adapt_backend(::HIPBackend, x::AbstractArray{<:Integer}) = _upload_index(collect(x))
 MagnetoPhotonic.adapt_backend(::HIPBackend, x) = Adapt.adapt(HIPArray, x)
 MagnetoPhotonic.to_host(x::HIPArray) = Array(x)
+MagnetoPhotonic.to_host(x::HIPIndexArray) = Int.(Array(x.data)) .+ 1
 MagnetoPhotonic.synchronize(::HIPBackend) = _check(ccall((:mp_synchronize, libmphip), Cint, ()), "mp_synchronize")
 MagnetoPhotonic.free_device!(x::HIPArray) = _free!(x)
+MagnetoPhotonic.free_device!(x::HIPIndexArray) = _free!(x.data)
 MagnetoPhotonic.reclaim_device_memory!(::HIPBackend) = nothing
 
 function MagnetoPhotonic.device_memory_info(::HIPBackend)
@@ -166,7 +205,38 @@ struct MpPlane
     energy::Ptr{Cvoid}
 end
 
+struct MpPulse
+    kind::Int32; _pad::Int32
+    amplitude::Cdouble; t0::Cdouble; tau::Cdouble; omega::Cdouble; phase::Cdouble
+end
+
+struct MpSource
+    kind::Int32; component::Int32; axis::Int32; _pad::Int32
+    i::Int64; j::Int64; k::Int64
+    i0::Int64; i1::Int64
+    w0::Cdouble; w1::Cdouble
+    profile::Ptr{Cvoid}
+    hcorr::Cdouble
+    pulse::MpPulse
+end
+
+struct MpState
+    yee::Ptr{MpYee}; cpml::Ptr{MpCpml}; ade::NTuple{3,Ptr{MpAde}}; mo::Ptr{MpMo}
+    mo_ea::Ptr{Cvoid}; mo_eb::Ptr{Cvoid}; mo_iea::Ptr{Cvoid}; mo_ieb::Ptr{Cvoid}
+    film::Ptr{MpFilm}; params::Ptr{FerrimagnetParameters{Float64}}; lut::Ptr{MpLut}
+    source::MpSource
+    planes::Ptr{MpPlane}; nplanes::Int32; nb::Int32
+    omega_bins::Ptr{Cdouble}; phase::Ptr{Cdouble}
+    dt::Cdouble; mu0::Cdouble; pabs_factor::Cdouble; Q_TM::Cdouble; Q_RE::Cdouble; t::Cdouble
+    n::Int64; last_mp_n::Int64
+    multiphysics_every::Int32; subcycles::Int32; brillouin_iters::Int32; multiphysics_enabled::Int32
+    has_ade_work::Int32; record_dft::Int32
+    surrogate::Ptr{MpMlp}; surrogate_tol::Cdouble; surrogate_fallbacks::Ptr{Int64}
+    film_lists_coincide::Int32; film_serial::Int32
+end
+
 _p(x::HIPArray) = Ptr{Cvoid}(x.ptr)
+_p(x::HIPIndexArray) = Ptr{Cvoid}(x.data.ptr)
 _p(::Nothing) = C_NULL
 
 function _yee(f::FieldState{T}, ie_x, ie_y, ie_z, dcx, dcy, dcz, ddx, ddy, ddz, ::Type{CT}) where {T,CT}
@@ -189,6 +259,75 @@ function _cpml(c)
     return MpCpml(c.Npml_x, c.Npml_y, c.Npml_z, _psi_dtype(c),
                   (_p(c.x.inv_kappa), _p(c.y.inv_kappa), _p(c.z.inv_kappa)),
                   (_p(c.x.a), _p(c.y.a), _p(c.z.a)), (_p(c.x.b), _p(c.y.b), _p(c.z.b)), ph, pd, 1.0)
+end
+
+# ---- reductions, scalar and plane downloads (src/core/Backend.jl:49-83) --------------------------
+const MP_RED_SUM, MP_RED_COUNT_NEGATIVE, MP_RED_NORM_SUM = Int32(0), Int32(1), Int32(2)
+
+function _reduce(op::Int32, x::HIPArray{T}, y=nothing, z=nothing) where {T<:Union{Float32,Float64}}
+    out = HIPArray{Float64,1}((1,))
+    _check(ccall((:mp_reduce, libmphip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int64, Int32, Ptr{Cvoid}),
+                 _p(x), _p(y), _p(z), _dtype(T), Int64(length(x)), op, _p(out)), "mp_reduce")
+    r = Array(out)[1]
+    _free!(out)
+    return r
+end
+
+MagnetoPhotonic.reduce_sum(x::HIPArray) = isempty(x) ? 0.0 : _reduce(MP_RED_SUM, x)
+MagnetoPhotonic.reduce_mean(x::HIPArray) = isempty(x) ? NaN : _reduce(MP_RED_SUM, x) / length(x)
+MagnetoPhotonic.reduce_count_negative(x::HIPArray) = isempty(x) ? 0 : Int(_reduce(MP_RED_COUNT_NEGATIVE, x))
+function MagnetoPhotonic.reduce_norm_mean(x::HIPArray{T}, y::HIPArray{T}, z::HIPArray{T}) where {T}
+    isempty(x) && return NaN
+    (length(y) == length(x) && length(z) == length(x)) || throw(DimensionMismatch("reduce_norm_mean lengths differ"))
+    return _reduce(MP_RED_NORM_SUM, x, y, z) / length(x)
+end
+
+function MagnetoPhotonic.any_nonfinite(x::HIPArray{T}) where {T<:Union{Float32,Float64}}
+    isempty(x) && return false
+    flag = HIPArray{Int32,1}((1,))
+    _check(ccall((:mp_memset, libmphip), Cint, (Ptr{Cvoid}, Cint, Csize_t), _p(flag), 0, sizeof(Int32)), "mp_memset")
+    _check(ccall((:mp_any_nonfinite, libmphip), Cint, (Ptr{Cvoid}, Int32, Int64, Ptr{Cvoid}),
+                 _p(x), _dtype(T), Int64(length(x)), _p(flag)), "mp_any_nonfinite")
+    r = Array(flag)[1] != 0
+    _free!(flag)
+    return r
+end
+
+function MagnetoPhotonic.scalar_to_host(x::HIPArray{T}, inds::Integer...) where {T}
+    li = LinearIndices(x)[Int.(inds)...]
+    out = Ref{T}()
+    _check(ccall((:mp_memcpy_d2h, libmphip), Cint, (Ptr{T}, Ptr{Cvoid}, Csize_t),
+                 out, Ptr{Cvoid}(x.ptr + (li - 1) * sizeof(T)), sizeof(T)), "mp_memcpy_d2h")
+    return out[]
+end
+
+# plane_to_host: the plane in the order of the reference's views (x -> field[i,:,:],
+# y -> field[:,i,:], z -> field[:,:,i]); a 2-D field is a 3-D one with one z plane.
+function MagnetoPhotonic.plane_to_host(field::HIPArray{T,N}, axis::Symbol, idx::Integer) where {T,N}
+    N == 1 && (axis === :x || throw(ArgumentError("1-D field plane axis must be :x")))
+    N == 1 && return [MagnetoPhotonic.scalar_to_host(field, idx)]
+    N in (2, 3) || throw(ArgumentError("field planes require a 1-D, 2-D, or 3-D array"))
+    nx, ny = size(field, 1), size(field, 2)
+    nz = N == 3 ? size(field, 3) : 1
+    a = axis === :x ? Int32(0) : axis === :y ? Int32(1) : (axis === :z && N == 3) ? Int32(2) :
+        throw(ArgumentError(N == 3 ? "3-D field plane axis must be :x, :y, or :z" : "2-D field plane axis must be :x or :y"))
+    dims = a == 0 ? (ny, nz) : a == 1 ? (nx, nz) : (nx, ny)
+    out = HIPArray{T,2}(dims)
+    _check(ccall((:mp_plane_gather, libmphip), Cint, (Ptr{Cvoid}, Int32, Int64, Int64, Int64, Int32, Int64, Ptr{Cvoid}),
+                 _p(field), _dtype(T), nx, ny, nz, a, Int64(idx) - 1, _p(out)), "mp_plane_gather")
+    r = Array(out)
+    _free!(out)
+    return N == 3 ? r : vec(r)
+end
+
+# sum(work_trace; dims=(1, 2)) of the probe trace buffer (ProbeReadout.jl:291): the sums
+# over the leading dimensions, returned as a device array of size (1, 1, n3).
+function Base.sum(a::HIPArray{Float64,3}; dims)
+    Tuple(dims) == (1, 2) || throw(ArgumentError("sum of a HIPArray supports dims=(1, 2)"))
+    out = HIPArray{Float64,3}((1, 1, size(a, 3)))
+    _check(ccall((:mp_sum_blocks, libmphip), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Cvoid}),
+                 _p(a), Int64(size(a, 1) * size(a, 2)), Int64(size(a, 3)), _p(out)), "mp_sum_blocks")
+    return out
 end
 
 # ---- launchers (src/fdtd/Kernels.jl:522-699) ---------------------------------------------------
@@ -384,6 +523,186 @@ function probe_shot_reductions(m::MagnetoPhotonic.ProbeReadout)
     j(i) = complex(r[4nb + 2 + 2i - 1], r[4nb + 2 + 2i]) * inv
     return (; Sx_pre, Sx_post, net_pre=r[4nb + 1], net_post=r[4nb + 2],
             jEy_far=j(1), jEz_far=j(2), jEy_kerr=j(3), jEz_kerr=j(4))
+end
+
+# probe_trace_plane_kernel! (Kernels.jl:467-478): the four mode-weighted plane samples into
+# work_trace (Ny, Nz, 4); ProbeReadout.jl:291 then sums them with sum(; dims=(1, 2)) above.
+function MagnetoPhotonic._ka_probe_trace_plane!(::HIPBackend, work_trace::HIPArray{Float64,3}, fields,
+                                                mode_w, area_yz, ix_pre::Integer, ix_post::Integer)
+    (size(work_trace, 1) > 0 && size(work_trace, 2) > 0) || return nothing
+    y = Ref(_yee(fields, nothing, nothing, nothing, nothing, nothing, nothing, nothing, nothing, nothing,
+                 eltype(fields.Ex)))
+    _check(ccall((:mp_probe_trace_samples, libmphip), Cint,
+                 (Ref{MpYee}, Int64, Int64, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                 y, Int64(ix_pre) - 1, Int64(ix_post) - 1, _p(mode_w), _p(area_yz), _p(work_trace)),
+           "mp_probe_trace_samples")
+    return nothing
+end
+
+# inject!(::FieldState, ::Grid3D, ::PlaneSource) (Source.jl:188-236) is a host loop over
+# the plane's cells in the reference; on device fields it is one mp_inject_plane_3d call
+# with the same _axis_weights (E += w v and D += w v / inv_eps formed in FP64).
+function MagnetoPhotonic.inject!(fields::FieldState{T,<:HIPArray}, grid::Grid3D, src::PlaneSource, t::Real,
+                                 p::FDTDParams, inv_eps) where {T}
+    ax = src.axis === :x ? Int32(0) : src.axis === :y ? Int32(1) : src.axis === :z ? Int32(2) :
+         throw(ArgumentError("3-D PlaneSource axis must be :x, :y or :z"))
+    w = MagnetoPhotonic._axis_weights(getfield(grid, src.axis), src.position)
+    i1, w1 = length(w) > 1 ? (Int64(w[2][1]) - 1, Float64(w[2][2])) : (Int64(-1), 0.0)
+    y = Ref(_yee(fields, inv_eps.inv_eps_x, inv_eps.inv_eps_y, inv_eps.inv_eps_z, nothing, nothing, nothing,
+                 nothing, nothing, nothing, T))
+    _check(ccall((:mp_inject_plane_3d, libmphip), Cint,
+                 (Ref{MpYee}, Int32, Int32, Int64, Cdouble, Int64, Cdouble, Cdouble),
+                 y, MagnetoPhotonic._component_code(src.component), ax, Int64(w[1][1]) - 1, Float64(w[1][2]), i1, w1,
+                 Float64(MagnetoPhotonic.source_value(src.pulse, t))), "mp_inject_plane_3d")
+    return fields
+end
+
+# ---- the whole step as one mp_run_steps call (Solver.jl:277-383, 557-563) ----------------------
+_pulse(p::GaussianPulse) = MpPulse(1, 0, p.amplitude, p.t0, p.tau, p.omega, p.phase)
+_pulse(p::CarrierGaussianPulse) = MpPulse(2, 0, p.amplitude, p.t0, p.sigma, p.omega, 0.0)
+_pulse(p::ContinuousSource) = MpPulse(3, 0, p.amplitude, 0.0, 1.0, p.omega, p.phase)
+_pulse(::Any) = nothing
+
+const _NO_PULSE = MpPulse(0, 0, 0.0, 0.0, 1.0, 0.0, 0.0)
+
+# The source of _step_typed! (Solver.jl:296-311) as an mp_source; nothing when the step
+# driver has no such source (the launcher path then runs the step).
+function _mp_source(state::FDTDState, source)
+    source === nothing && return MpSource(0, 0, 0, 0, 0, 0, 0, 0, -1, 0.0, 0.0, C_NULL, 0.0, _NO_PULSE)
+    if source isa Tuple
+        pulse, component, index = source
+        pl = _pulse(pulse)
+        pl === nothing && return nothing
+        return MpSource(1, MagnetoPhotonic._component_code(component), 0, 0, index[1] - 1, index[2] - 1,
+                        index[3] - 1, 0, -1, 0.0, 0.0, C_NULL, 0.0, pl)
+    elseif source isa PlaneSource
+        pl = _pulse(source.pulse)
+        (pl === nothing || !(source.axis in (:x, :y, :z))) && return nothing
+        ax = source.axis === :x ? 0 : source.axis === :y ? 1 : 2
+        w = MagnetoPhotonic._axis_weights(getfield(state.grid, source.axis), source.position)
+        i1, w1 = length(w) > 1 ? (w[2][1] - 1, Float64(w[2][2])) : (-1, 0.0)
+        return MpSource(2, MagnetoPhotonic._component_code(source.component), ax, 0, 0, 0, 0, w[1][1] - 1, i1,
+                        Float64(w[1][2]), w1, C_NULL, 0.0, pl)
+    elseif source isa ModeSource
+        pl = _pulse(source.pulse)
+        (pl === nothing || source.axis !== :x || !(source.component in (:Ey, :Ez))) && return nothing
+        i = MagnetoPhotonic._source_index(state.grid.x, source.position)
+        eta0 = sqrt(FDTDParams().mu0 / FDTDParams().eps0)   # as _ka_inject_mode_x_3d! (Kernels.jl:584)
+        return MpSource(3, MagnetoPhotonic._component_code(source.component), 0, 0, i - 1, 0, 0, 0, -1, 0.0, 0.0,
+                        _p(source.profile), source.neff / eta0, pl)
+    end
+    return nothing
+end
+
+# Whether every ADE list and the MO list is the film list (entry n is film cell n) and the
+# ADE state is in the compute type: the step then runs its list work as one fused pass.
+# Decided once per state from host copies of the index maps.
+const _COINCIDE = WeakKeyDict{FDTDState,Bool}()
+function _lists_coincide(state::FDTDState)
+    get!(_COINCIDE, state) do
+        r = state.region
+        (state.ade_x === nothing || state.thermal === nothing || r === nothing) && return false
+        eltype(state.ade_x.P) == state.compute_T || return false
+        cells = MagnetoPhotonic.to_host(r.material_cells)
+        ident = collect(1:length(cells))
+        for (idx, pos) in ((r.ade_idx_x, r.pos_x), (r.ade_idx_y, r.pos_y), (r.ade_idx_z, r.pos_z))
+            (MagnetoPhotonic.to_host(idx) == cells && MagnetoPhotonic.to_host(pos) == ident) || return false
+        end
+        if state.mo !== nothing
+            (MagnetoPhotonic.to_host(r.mo_idx) == cells && MagnetoPhotonic.to_host(state.mo_pos) == ident) ||
+                return false
+        end
+        return true
+    end
+end
+
+# One mp_run_steps call of `steps` steps on the state; false when the state holds a part
+# the step driver does not take (then nothing ran).
+function _run_steps!(state::FDTDState, steps::Integer)
+    CT = state.compute_T
+    src = _mp_source(state, state.source)
+    src === nothing && return false
+    r = state.region
+    yee = Ref(_yee(state.fields, state.inv_eps_x, state.inv_eps_y, state.inv_eps_z, state.inv_d_cell_x,
+                   state.inv_d_cell_y, state.inv_d_cell_z, state.inv_d_dual_x, state.inv_d_dual_y,
+                   state.inv_d_dual_z, CT))
+    cpml = Ref(_cpml(state.cpml))
+    ade = state.ade_x === nothing ? nothing :
+          (Ref(_ade(state.ade_x, r.ade_idx_x, r.ade_f_x, r.ade_inv_x, state.diag_poles, CT)),
+           Ref(_ade(state.ade_y, r.ade_idx_y, r.ade_f_y, r.ade_inv_y, state.diag_poles, CT)),
+           Ref(_ade(state.ade_z, r.ade_idx_z, r.ade_f_z, r.ade_inv_z, state.diag_poles, CT)))
+    film_on = state.thermal !== nothing && state.mag !== nothing && state.absorption !== nothing
+    mo_on = state.mo !== nothing && state.mag !== nothing
+    mo = mo_on ? Ref(MpMo(length(r.mo_idx), length(state.mo_poles), _dtype(CT), _p(state.mo.Py_from_z),
+                          _p(state.mo.Jy_from_z), _p(state.mo.Pz_from_y), _p(state.mo.Jz_from_y),
+                          _p(state.mo.E_old_y), _p(state.mo.E_old_z), _p(r.mo_idx), _p(r.mo_fill),
+                          _p(state.mo_pos), _p(state.mo_poles))) : nothing
+    film = film_on ? Ref(_film(state.thermal, state.mag, state.absorption, r, CT)) : nothing
+    gd = state.model isa MagnetoOpticModel ? state.model.params : nothing
+    params = gd === nothing ? nothing :
+             Ref(FerrimagnetParameters{Float64}(ntuple(i -> Float64(getfield(gd, i)), fieldcount(typeof(gd)))...))
+    lut = nothing
+    if film_on
+        tm_lut, re_lut, T_min, inv_dT, lut_N = state.lut
+        lut = Ref(MpLut(_p(tm_lut), _p(re_lut), Float64(T_min), Float64(inv_dT), Int32(lut_N), _dtype(CT)))
+    end
+    sur = SURROGATE[]
+    mlp = sur === nothing ? nothing : Ref(sur[1])
+    hw = state.absorption_model === :ade_work && state.ade_x !== nothing && length(state.ade_x.J) > 0
+    pabs_factor = gd === nothing ? 0.0 :
+                  0.5 * Float64(gd.omega_pump) * Float64(state.params.eps0) * Float64(gd.eps_imag_pump)
+    ptr(x) = x === nothing ? C_NULL : Base.unsafe_convert(Ptr{eltype(x)}, x)
+    GC.@preserve yee cpml ade mo film params lut mlp state begin
+        st = Ref(MpState(ptr(yee), ptr(cpml),
+                         ade === nothing ? (Ptr{MpAde}(C_NULL), Ptr{MpAde}(C_NULL), Ptr{MpAde}(C_NULL)) :
+                                           (ptr(ade[1]), ptr(ade[2]), ptr(ade[3])),
+                         mo_on ? ptr(mo) : Ptr{MpMo}(C_NULL),
+                         mo_on ? _p(state.fields.Ey) : C_NULL, mo_on ? _p(state.fields.Ez) : C_NULL,
+                         mo_on ? _p(state.inv_eps_y) : C_NULL, mo_on ? _p(state.inv_eps_z) : C_NULL,
+                         film_on ? ptr(film) : Ptr{MpFilm}(C_NULL),
+                         params === nothing ? Ptr{FerrimagnetParameters{Float64}}(C_NULL) : ptr(params),
+                         film_on ? ptr(lut) : Ptr{MpLut}(C_NULL),
+                         src, Ptr{MpPlane}(C_NULL), Int32(0), Int32(0), Ptr{Cdouble}(C_NULL), Ptr{Cdouble}(C_NULL),
+                         state.dt, Float64(state.params.mu0), pabs_factor,
+                         gd === nothing ? 0.0 : Float64(gd.Q_voigt_TM), gd === nothing ? 0.0 : Float64(gd.Q_voigt_RE),
+                         state.t, Int64(state.n), Int64(state.last_mp_n),
+                         Int32(state.multiphysics_every), Int32(state.subcycles), Int32(state.brillouin_iters),
+                         Int32(state.multiphysics_enabled && film_on), Int32(hw), Int32(0),
+                         mlp === nothing ? Ptr{MpMlp}(C_NULL) : ptr(mlp), sur === nothing ? 0.0 : sur[2],
+                         sur === nothing ? Ptr{Int64}(C_NULL) : sur[3],
+                         Int32(film_on && _lists_coincide(state)), Int32(0)))
+        _check(ccall((:mp_run_steps, libmphip), Cint, (Ref{MpState}, Int64), st, Int64(steps)), "mp_run_steps")
+        state.n = Int(st[].n)
+        state.t = st[].t
+        state.last_mp_n = Int(st[].last_mp_n)
+    end
+    return true
+end
+
+# step! on this backend (Solver.jl:277-369): the fixed sequence in one mp_run_steps call
+# (H, source, E, the fused film pass or the list kernels, absorbed power, the film update
+# when its window closes).  A state with a source the step driver does not take (a point
+# source with trilinear weights, a non-x mode source) runs the reference's launcher
+# sequence instead, every launch of which is a method of this extension.
+function MagnetoPhotonic._step_typed!(state::FDTDState, backend::HIPBackend, ::Type{CT},
+                                      inv_eps_x, inv_eps_y, inv_eps_z, inv_d_cell_x, inv_d_cell_y, inv_d_cell_z,
+                                      inv_d_dual_x, inv_d_dual_y, inv_d_dual_z,
+                                      source, region, diag_poles, mo_poles, mo_pos, model) where {CT}
+    _run_steps!(state, 1) && return state
+    return invoke(MagnetoPhotonic._step_typed!,
+                  Tuple{FDTDState,Any,Type{CT},Any,Any,Any,Any,Any,Any,Any,Any,Any,Any,Any,Any,Any,Any,Any},
+                  state, backend, CT, inv_eps_x, inv_eps_y, inv_eps_z, inv_d_cell_x, inv_d_cell_y, inv_d_cell_z,
+                  inv_d_dual_x, inv_d_dual_y, inv_d_dual_z, source, region, diag_poles, mo_poles, mo_pos, model)
+end
+
+# run!(state, steps) (Solver.jl:557-563) without a callback: every step in one
+# mp_run_steps call, so each step's film update overlaps the next step's Yee updates.
+# Other backends, or a callback, take the reference's loop.
+function MagnetoPhotonic.run!(state::FDTDState, steps::Int; callback=nothing)
+    if callback === nothing && state.backend isa HIPBackend && steps > 0 && _run_steps!(state, steps)
+        return state
+    end
+    return invoke(MagnetoPhotonic.run!, Tuple{FDTDState,Integer}, state, steps; callback=callback)
 end
 
 function __init__()

@@ -32,6 +32,7 @@ def lib():
         abi.bind(h, "mp_", abi.SIGNATURES)
         abi.bind(h, "mp_", abi.DRIVER_SIGNATURES)
         abi.bind(h, "mp_", abi.NOALLOC_SIGNATURES)
+        abi.bind(h, "mp_", abi.HOOK_SIGNATURES)
         abi.bind(h, "", abi.RUNTIME_SIGNATURES)
         abi.bind(h, "", abi.SURROGATE_SIGNATURES)
         _lib = h

@@ -122,6 +122,15 @@ SIGNATURES = {
     "last_error": (C.c_char_p, []),
 }
 
+# Device reductions and plane copies behind the backend hooks (checked against numpy).
+MP_RED_SUM, MP_RED_COUNT_NEGATIVE, MP_RED_NORM_SUM = 0, 1, 2
+HOOK_SIGNATURES = {
+    "reduce": (C.c_int, [vp, vp, vp, i32, i64, i32, vp]),
+    "sum_blocks": (C.c_int, [vp, i64, i64, vp]),
+    "plane_gather": (C.c_int, [vp, i32, i64, i64, i64, i32, i64, vp]),
+    "probe_trace_samples": (C.c_int, [P(Yee), i64, i64, vp, vp, vp]),
+}
+
 # No-allocation variants of entry points (fixed library scratch); no oracle counterpart.
 NOALLOC_SIGNATURES = {
     "probe_shot_reduce_noalloc": (C.c_int, [P(Plane), P(Plane), i64, i64, i32, i32, vp, vp, vp]),
