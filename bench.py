@@ -9,7 +9,11 @@ over the 512 x 512 x 256 synthetic pump workload, inputs resident in HBM.  Print
 JSON line: Mcell-updates/s = Nx*Ny*Nz*K / wall, the roofline of the dominant kernel
 (algorithmic bytes per launch / mean launch time from HIP events recorded around each of
 its launches in the timed region; the per-kernel breakdown comes from further steps with
-every launch bracketed) and the CPU oracle timed on the host cores.
+every launch bracketed, then from as many steps with the film update serial, each kernel
+alone), the FP64 roofline of the film update, and the CPU oracle timed on the host cores
+(FP32 on every core, FP32 on one thread, FP64 on every core).  Counter figures (measured
+HBM traffic, FP64 operations) are printed only while the kernel sources have the digest
+they were measured on, else null.
 
 --form mixed stores CPML psi and the ADE state in BF16.  --form surrogate replaces the
 4TM + LLB substeps with the learned film update (checked against plain substeps, plain
@@ -27,20 +31,21 @@ sys.path.insert(0, os.path.join(ROOT, "magnetophotonic.jl_amd"))
 
 METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Δm|≤1e-3"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
 KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "film_epilogue", "multiphysics",
            "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate", "dft_snapshot", "dft_batch")
 DFT_BATCH = 32   # steps per batched DFT pass in the library
 
 
-def build_workload(name: str, form: str):
+def build_workload(name: str, form: str, dtype: str = "f32"):
     from mphip import configs
     if name == "coupled_pump":
         kw = {}
         if form == "mixed":
             kw = dict(psi_precision="bf16", ade_state_precision="bf16")
-        spec, meta = configs.coupled_pump(nx=512, ny=512, nz=256, steps=20000, **kw)
+        spec, meta = configs.coupled_pump(nx=512, ny=512, nz=256, steps=20000, dtype=dtype, **kw)
     elif name == "em_stencil":
-        spec, meta = configs.em_stencil(n=384, npml=16, steps=5000)
+        spec, meta = configs.em_stencil(n=384, npml=16, steps=5000, dtype=dtype)
         meta["planes"], meta["omega_bins"] = [], []
     else:
         raise SystemExit(f"unknown workload {name}")
@@ -87,25 +92,51 @@ def relax_accuracy():
     return res
 
 
-def cpu_baseline(spec, meta, steps: int, warmup: int = 1):
-    """The C oracle (a restatement of the reference CPU path) on the host cores."""
+def _oracle_rate(driver, spec, max_steps: int, budget_s: float, warmup: int = 1):
+    """Steps of the C oracle on `spec` after `warmup` steps: up to max_steps, stopping once
+    budget_s has passed (at least one).  Returns (steps, seconds)."""
+    st = driver.OracleState(spec)
+    for _ in range(warmup):
+        st.step()
+    k, t0 = 0, time.perf_counter()
+    while k < max_steps and (k == 0 or time.perf_counter() - t0 < budget_s):
+        st.step()
+        k += 1
+    return k, time.perf_counter() - t0
+
+
+def cpu_baseline(spec, meta, workload: str, steps: int, budget_s: float = 12.0):
+    """The C oracle (a restatement of the reference CPU path) on the host cores: FP32 on
+    every core (the headline figure), FP32 on one thread, FP64 on every core.  Each sample
+    is bounded (steps or seconds, whichever ends first) after one warm-up step."""
+    import ctypes as C
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import driver
     from mphip import abi
     L = driver.lib(abi)
-    L.orc_num_threads.restype = __import__("ctypes").c_int
-    st = driver.OracleState(spec)
-    for _ in range(warmup):
-        st.step()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        st.step()
-    el = time.perf_counter() - t0
+    L.orc_num_threads.restype = C.c_int
+    L.orc_set_num_threads.restype = None
+    L.orc_set_num_threads.argtypes = [C.c_int]
     nx, ny, nz = spec.shape
-    return dict(value=round(nx * ny * nz * steps / el / 1e6, 2), unit="Mcell-updates/s",
-                cores=int(L.orc_num_threads()), kind="port",
-                sample=f"{steps} steps of {meta['name']} after {warmup} warm-up step(s), FP32, "
-                       f"C oracle with OpenMP, {el:.1f} s")
+    threads = int(L.orc_num_threads())
+
+    def entry(k, el, cores, what):
+        return dict(value=round(nx * ny * nz * k / el / 1e6, 2), unit="Mcell-updates/s", cores=cores,
+                    kind="port", sample=f"{k} steps of {meta['name']} after 1 warm-up step, {what}, "
+                                        f"C oracle, {el:.1f} s")
+
+    k, el = _oracle_rate(driver, spec, steps, 1e9)
+    out = entry(k, el, threads, f"FP32, OpenMP on {threads} threads")
+    L.orc_set_num_threads(1)
+    try:
+        k, el = _oracle_rate(driver, spec, steps, budget_s)
+    finally:
+        L.orc_set_num_threads(threads)
+    out["one_thread"] = entry(k, el, 1, "FP32, one thread")
+    spec64, _ = build_workload(workload, "plain", dtype="f64")
+    k, el = _oracle_rate(driver, spec64, steps, budget_s)
+    out["fp64"] = entry(k, el, threads, f"FP64, OpenMP on {threads} threads")
+    return out
 
 
 def main():
@@ -196,6 +227,22 @@ def main():
         if n.value:
             stats[k] = dict(launches=n.value, avg_us=round(ms.value / n.value * 1e3, 2))
     L.mp_profile_enable(0)
+    # the same steps with the film update on the main stream after the step's launches:
+    # every kernel's time alone, without the side stream's company
+    serial = {}
+    if spec.has_film:
+        st.film_serial = True
+        L.mp_profile_select(None)
+        L.mp_profile_enable(1)
+        st.run(brk_steps)
+        st.synchronize()
+        for k in KERNELS:
+            n, ms = C.c_int64(), C.c_double()
+            L.mp_kernel_stats(k.encode(), C.byref(n), C.byref(ms))
+            if n.value:
+                serial[k] = dict(launches=n.value, avg_us=round(ms.value / n.value * 1e3, 2))
+        L.mp_profile_enable(0)
+        st.film_serial = False
     mcell = ncell * args.steps / wall / 1e6
     per_launch = kb[dom] / (dom_launches / args.steps)
     achieved = per_launch / (dom_avg_us * 1e-6) / 1e9
@@ -203,7 +250,25 @@ def main():
         if k in kb:
             launches_per_step = v["launches"] / brk_steps
             v["alg_GBs"] = round(kb[k] / launches_per_step / (v["avg_us"] * 1e-6) / 1e9, 1)
+    for k, v in serial.items():
+        if k in kb:
+            v["alg_GBs"] = round(kb[k] / (v["launches"] / brk_steps) / (v["avg_us"] * 1e-6) / 1e9, 1)
     step_gbs = kb["total"] / (wall / args.steps) / 1e9
+    traffic, traffic_src = roofline.measured_traffic(meta["name"], args.form, dom, eps_idx)
+    film_rf = None
+    if "multiphysics" in stats and args.form != "surrogate":
+        ops, ops_src = roofline.fp64_ops_per_cell_substep(meta["name"])
+        flops = ops * spec.n_film * spec.subcycles if ops else None
+        launch_us = {"beside the Yee updates": stats["multiphysics"]["avg_us"]}
+        if "multiphysics" in serial:
+            launch_us["alone"] = serial["multiphysics"]["avg_us"]
+        film_rf = {"bound": "fp64", "kernel": "multiphysics", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                   "ops_per_cell_substep": ops, "flops_per_launch": flops, "ops_from": ops_src,
+                   "avg_us": launch_us,
+                   "achieved": {k: (round(flops / (u * 1e-6) / 1e12, 2) if flops else None)
+                                for k, u in launch_us.items()},
+                   "frac": {k: (round(flops / (u * 1e-6) / 1e12 / FP64_PEAK_TFLOPS, 4) if flops else None)
+                            for k, u in launch_us.items()}}
     out = {
         "metric": METRIC,
         "value": round(mcell, 1),
@@ -227,7 +292,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "launches": dom_launches, "avg_us": round(dom_avg_us, 2),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": roofline.measured_traffic(meta["name"], args.form, dom, eps_idx),
+                     "traffic": traffic, "traffic_from": traffic_src,
                      "alg_bytes_per_launch": int(per_launch)},
         "step_roofline": {"alg_bytes_per_step": kb["total"],
                           "alg_bytes_per_cell": round(kb["total"] / ncell, 2),
@@ -235,6 +300,12 @@ def main():
         "kernels": stats,
         "kernels_from": f"{brk_steps} steps after the timed region, every launch bracketed by HIP events",
     }
+    if serial:
+        out["kernels_film_serial"] = serial
+        out["kernels_film_serial_from"] = (f"{brk_steps} further steps with the film update on the main stream "
+                                           f"after the step's launches (each kernel alone)")
+    if film_rf is not None:
+        out["film_roofline"] = film_rf
     if sur_info is not None:
         sur_info["tol"] = args.tol
         sur_info["fallback_frac"] = round(fallbacks / (spec.n_film * args.steps), 5)
@@ -242,7 +313,7 @@ def main():
             sur_info["relax_accuracy"] = relax_accuracy()
         out["config"]["surrogate"] = sur_info
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(spec, meta, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(spec, meta, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,9 @@ behind the roofline fractions the benchmark reports.
 """
 from __future__ import annotations
 
+import hashlib
+import os
+
 import numpy as np
 
 from .cpml import PSI_AXIS
@@ -103,15 +106,54 @@ def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch:
     return kb
 
 
-# HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE x 2.0 + WRITE_SIZE,
-# KiB per dispatch; the factors are what the counters read for a plain copy of known size
-# on gfx950), in runs separate from the timed one: profiles/yee_E_coupled_pump_512x512x256_pmc.md.
+# Counter measurements of the kernels, each valid only for the kernel sources it was taken
+# on: `digest` is source_digest() of those files at the time, and a lookup whose current
+# digest differs returns nothing, so a figure from older kernels is never reported as one
+# of the running build.
+YEE_SOURCES = ("yee.hip", "common.h", "Makefile")
+FILM_SOURCES = ("thermo.hip", "film_physics.h", "common.h", "Makefile")
+
+_CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc")
+
+
+def source_digest(names) -> str:
+    """First 16 hex digits of the sha256 of the named csrc files, concatenated in order."""
+    h = hashlib.sha256()
+    for n in names:
+        with open(os.path.join(_CSRC, n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+# HBM bytes per launch from rocprofv3 PMC runs (FETCH_SIZE x 2.0 + WRITE_SIZE, KiB per
+# dispatch; the factors are what the counters read for a plain 1 GiB device copy on gfx950).
 MEASURED_TRAFFIC = {
-    ("coupled_pump_512x512x256", "plain", "index"): {"yee_E": 4133190159, "yee_H": 3222221392},
+    ("coupled_pump_512x512x256", "plain", "index"): {
+        "digest": "1da1f41030312908", "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md",
+        "bytes": {"yee_E": 4133415722, "yee_H": 3222247005}},
+}
+
+# FP64 operations of one film cell-substep of the 4TM + LLB update from SQ_INSTS_VALU_*_F64
+# counters (2 x FMA + ADD + MUL, x 64 lanes, per dispatch / (cells x substeps)).
+FP64_OPS = {
+    "coupled_pump_512x512x256": {"digest": "cab2f6c1d5338377", "ops": 1143.4,
+                                 "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md"},
 }
 
 
 def measured_traffic(workload: str, form: str, kernel: str, eps_index: bool = False):
-    """Measured HBM bytes per launch of `kernel`, or None where none was taken."""
-    key = (workload, form, "index" if eps_index else "volumes")
-    return MEASURED_TRAFFIC.get(key, {}).get(kernel)
+    """(bytes per launch of `kernel`, profile file) measured on the current Yee sources, or
+    (None, None) where none was."""
+    m = MEASURED_TRAFFIC.get((workload, form, "index" if eps_index else "volumes"))
+    if not m or kernel not in m["bytes"] or m["digest"] != source_digest(YEE_SOURCES):
+        return None, None
+    return m["bytes"][kernel], m["profile"]
+
+
+def fp64_ops_per_cell_substep(workload: str):
+    """(FP64 operations per film cell-substep, profile file) measured on the current film
+    sources, or (None, None)."""
+    m = FP64_OPS.get(workload)
+    if not m or m["digest"] != source_digest(FILM_SOURCES):
+        return None, None
+    return m["ops"], m["profile"]

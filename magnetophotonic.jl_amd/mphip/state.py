@@ -413,6 +413,17 @@ class FDTDState:
         self._st.nplanes = 0
         self._dft = None
 
+    @property
+    def film_serial(self) -> bool:
+        """The film update runs on the main stream after the step's launches instead of on
+        the side stream beside the next step's Yee updates (same results)."""
+        return bool(self._st.film_serial)
+
+    @film_serial.setter
+    def film_serial(self, on: bool) -> None:
+        self._film_serial = bool(on)
+        self._st.film_serial = 1 if on else 0
+
     def step(self) -> "FDTDState":
         """One EM step (step!, Solver.jl:277-369)."""
         return self.run(1)

@@ -151,6 +151,8 @@ static int orc_fail(const char *msg) {
 }
 ORC_EXPORT const char *orc_last_error(void) { return orc_err; }
 ORC_EXPORT int orc_num_threads(void) { return omp_get_max_threads(); }
+/* threads of the OpenMP loops from here on (n <= 0: unchanged) */
+ORC_EXPORT void orc_set_num_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 /* element-wise storage conversions of the reduced kinds, exported for their own tests:
  * out[i] = bits of x[i]*sc, and back, y[i] = value(h[i])*isc */

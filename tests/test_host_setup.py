@@ -78,6 +78,26 @@ def test_roofline_bulk_bytes():
     assert kb["yee_E"] - 60 * n_upd == kb["yee_H"] - 36 * n_upd   # same psi slab traffic
 
 
+def test_measured_figures_follow_the_kernel_sources(monkeypatch):
+    """Counter figures are reported only for the kernel sources they were measured on: an
+    entry whose digest is not the current sources' gives (None, None)."""
+    from mphip import roofline
+    cur_y = roofline.source_digest(roofline.YEE_SOURCES)
+    cur_f = roofline.source_digest(roofline.FILM_SOURCES)
+    assert len(cur_y) == 16 and cur_y != cur_f
+    key = ("w", "plain", "index")
+    monkeypatch.setitem(roofline.MEASURED_TRAFFIC, key, {"digest": cur_y, "profile": "p.md", "bytes": {"yee_E": 7}})
+    assert roofline.measured_traffic("w", "plain", "yee_E", True) == (7, "p.md")
+    assert roofline.measured_traffic("w", "plain", "yee_E", False) == (None, None)
+    assert roofline.measured_traffic("w", "plain", "yee_H", True) == (None, None)
+    monkeypatch.setitem(roofline.MEASURED_TRAFFIC, key, {"digest": "0" * 16, "profile": "p.md", "bytes": {"yee_E": 7}})
+    assert roofline.measured_traffic("w", "plain", "yee_E", True) == (None, None)
+    monkeypatch.setitem(roofline.FP64_OPS, "w", {"digest": cur_f, "ops": 3.0, "profile": "q.md"})
+    assert roofline.fp64_ops_per_cell_substep("w") == (3.0, "q.md")
+    monkeypatch.setitem(roofline.FP64_OPS, "w", {"digest": cur_y, "ops": 3.0, "profile": "q.md"})
+    assert roofline.fp64_ops_per_cell_substep("w") == (None, None)
+
+
 def test_em_stencil_config_small():
     from mphip import configs
     spec, meta = configs.em_stencil(n=64, npml=8, steps=10, n_boxes=6, sphere_radius=6)
