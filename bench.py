@@ -1,7 +1,8 @@
 """Benchmark of the coupled 3-D time step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload coupled_pump|em_stencil]
-                    [--form plain|mixed|surrogate] [--no-cpu]
+                    [--form plain|mixed|surrogate|surrogate+mixed] [--storage bf16|f16]
+                    [--no-accuracy] [--no-cpu]
 
 A step is one pass of the full coupled sequence (Yee H, pump source, Yee E, ADE x/y/z,
 gyrotropic MO, absorbed power, 4TM + LLB with 4 substeps, DFT on 6 planes x 4 bins)
@@ -15,10 +16,16 @@ alone), the FP64 roofline of the film update, and the CPU oracle timed on the ho
 HBM traffic, FP64 operations) are printed only while the kernel sources have the digest
 they were measured on, else null.
 
---form mixed stores CPML psi and the ADE state in BF16.  --form surrogate replaces the
-4TM + LLB substeps with the learned film update (checked against plain substeps, plain
-fallback); its network is first trained on the GPU, outside the timed region, on rows
-from a 96 x 96 x 256 slice of the same workload.
+--form mixed stores CPML psi and the ADE state in BF16 (--storage f16: binary16 with
+scales).  --form surrogate replaces the 4TM + LLB substeps with the learned film update
+(checked against plain substeps, plain fallback); its network is first trained on the
+GPU, outside the timed region, on rows from a 96 x 96 x 256 slice of the same workload.
+--form surrogate+mixed is both.
+
+The default plain run also prints `form_comparison`, after the timed region: the
+surrogate + BF16 and surrogate + binary16 forms timed on the same step, and the final
+film-mean |dm| of each form on the 128 x 128 x 64 accuracy case against the same path in
+FP64 on the GPU (--no-accuracy skips it).
 """
 import argparse
 import json
@@ -37,12 +44,15 @@ KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs",
 DFT_BATCH = 32   # steps per batched DFT pass in the library
 
 
-def build_workload(name: str, form: str, dtype: str = "f32"):
+FORMS = ("plain", "mixed", "surrogate", "surrogate+mixed")
+
+
+def build_workload(name: str, form: str, dtype: str = "f32", storage: str = "bf16"):
     from mphip import configs
     if name == "coupled_pump":
         kw = {}
-        if form == "mixed":
-            kw = dict(psi_precision="bf16", ade_state_precision="bf16")
+        if form in ("mixed", "surrogate+mixed"):
+            kw = dict(psi_precision=storage, ade_state_precision=storage)
         spec, meta = configs.coupled_pump(nx=512, ny=512, nz=256, steps=20000, dtype=dtype, **kw)
     elif name == "em_stencil":
         spec, meta = configs.em_stencil(n=384, npml=16, steps=5000, dtype=dtype)
@@ -89,6 +99,44 @@ def relax_accuracy():
     res["rows"] = int(X.shape[0])
     res["reference"] = "same path in FP64 on the GPU"
     res["study_s"] = round(time.perf_counter() - t0, 1)
+    return res
+
+
+def form_comparison(plain_ms: float, steps: int, warmup: int, tol: float, kinds=("bf16", "f16")):
+    """The surrogate + reduced-storage forms against plain FP32: ms per step of the
+    512 x 512 x 256 pump step in each form (same DFT planes; the pump network trained on the
+    GPU first, outside the timed steps), and the final film-mean |dm| of the 128 x 128 x 64
+    accuracy case against the same path in FP64 on the GPU for each form."""
+    from mphip import _lib
+    from mphip import surrogate as S
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+    t0 = time.perf_counter()
+    res = {"plain": {"ms_per_step": round(plain_ms, 4)}}
+    sur, info = train_pump_surrogate(warmup + steps)
+    for kind in kinds:
+        spec, meta = build_workload("coupled_pump", "surrogate+mixed", storage=kind)
+        st = FDTDState(spec)
+        st.attach_dft(DFTPlanes(st, meta["planes"], meta["omega_bins"]))
+        st.attach_surrogate(sur, tol)
+        st.run(warmup)
+        st.synchronize()
+        sur.reset_fallbacks()
+        t1 = time.perf_counter()
+        st.run(steps)
+        st.synchronize()
+        ms = (time.perf_counter() - t1) / steps * 1e3
+        res[f"surrogate+{kind}"] = {"ms_per_step": round(ms, 4), "speedup_vs_plain": round(plain_ms / ms, 3),
+                                    "fallback_frac": round(sur.fallback_count() / (spec.n_film * steps), 5),
+                                    "film_list_work": "one fused pass" if st.fused_epilogue else "five launches"}
+        st.free()
+        _lib.call("synchronize")
+    sur.free()
+    res["pump_network"] = info
+    res["steps"], res["warmup"], res["tol"] = steps, warmup, tol
+    res["accuracy_128x128x64"] = S.accuracy_forms(kinds, tol)
+    res["target"] = "surrogate+mixed >= 2x plain at film-mean |dm| <= 1e-3 against FP64"
+    res["comparison_s"] = round(time.perf_counter() - t0, 1)
     return res
 
 
@@ -145,9 +193,13 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="coupled_pump")
-    ap.add_argument("--form", default="plain", choices=("plain", "mixed", "surrogate"))
+    ap.add_argument("--form", default="plain", choices=FORMS)
+    ap.add_argument("--storage", default="bf16", choices=("bf16", "f16"),
+                    help="psi / ADE storage of the mixed forms")
     ap.add_argument("--tol", type=float, default=1e-4, help="surrogate check tolerance")
-    ap.add_argument("--no-accuracy", action="store_true", help="surrogate form: skip the relax accuracy study")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the form comparison (plain) / the relax accuracy study (surrogate)")
+    ap.add_argument("--compare-steps", type=int, default=50, help="timed steps per form in the form comparison")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -172,19 +224,20 @@ def main():
         _lib.call("synchronize")
         a.free()
         b.free()
-    spec, meta = build_workload(args.workload, args.form)
+    spec, meta = build_workload(args.workload, args.form, storage=args.storage)
     planes, bins = meta.get("planes", []), meta.get("omega_bins", [])
     st = FDTDState(spec)
     if planes:
         st.attach_dft(DFTPlanes(st, planes, bins))
     L = _lib.lib()
     sur_info = None
-    if args.form == "surrogate":
+    surrogate_form = args.form.startswith("surrogate")
+    if surrogate_form:
         sur, sur_info = train_pump_surrogate(args.warmup + args.steps)
         st.attach_surrogate(sur, args.tol)
     st.run(args.warmup)
     st.synchronize()
-    if args.form == "surrogate":
+    if surrogate_form:
         sur.reset_fallbacks()
 
     nx, ny, nz = spec.shape
@@ -256,7 +309,7 @@ def main():
     step_gbs = kb["total"] / (wall / args.steps) / 1e9
     traffic, traffic_src = roofline.measured_traffic(meta["name"], args.form, dom, eps_idx)
     film_rf = None
-    if "multiphysics" in stats and args.form != "surrogate":
+    if "multiphysics" in stats and not surrogate_form:
         ops, ops_src = roofline.fp64_ops_per_cell_substep(meta["name"])
         flops = ops * spec.n_film * spec.subcycles if ops else None
         launch_us = {"beside the Yee updates": stats["multiphysics"]["avg_us"]}
@@ -280,8 +333,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"plain": "f32", "mixed": "f32 (bf16 psi/ADE storage)",
-                  "surrogate": "f32 (film update: bf16 MFMA network, f64 check)"}[args.form],
+        "dtype": {"plain": "f32", "mixed": f"f32 ({args.storage} psi/ADE storage)",
+                  "surrogate": "f32 (film update: bf16 MFMA network, f64 check)",
+                  "surrogate+mixed": f"f32 ({args.storage} psi/ADE storage; film update: bf16 MFMA network, "
+                                     f"f64 check)"}[args.form],
         "data": "synthetic",
         "config": {"workload": meta["name"], "grid": list(spec.shape), "film_cells": spec.n_film,
                    "form": args.form, "dft_planes": len(planes), "dft_bins": len(bins),
@@ -312,6 +367,10 @@ def main():
         if not args.no_accuracy:
             sur_info["relax_accuracy"] = relax_accuracy()
         out["config"]["surrogate"] = sur_info
+    if args.form == "plain" and args.workload == "coupled_pump" and not args.no_accuracy:
+        st.free()
+        out["form_comparison"] = form_comparison(wall / args.steps * 1e3, min(args.compare_steps, args.steps),
+                                                 min(args.warmup, 10), args.tol)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(spec, meta, args.workload, args.cpu_steps)
     print(json.dumps(out), flush=True)

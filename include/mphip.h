@@ -274,8 +274,8 @@ typedef struct mp_state {
     double surrogate_tol;
     int64_t *surrogate_fallbacks;
     /* Non-zero when every ADE list and the MO list are the film list (entry n is film
-     * cell n) and the ADE state is stored in the compute type: the step then runs the
-     * list work after the E update as one mp_film_epilogue pass. */
+     * cell n) and the three ADE lists share one storage form: the step then runs the list
+     * work after the E update as one mp_film_epilogue pass. */
     int32_t film_lists_coincide;
     /* Non-zero runs the film update on the library stream in step order; zero (default)
      * runs it on a side stream beside the next step's Yee updates, joined before the
@@ -381,8 +381,9 @@ int mp_pabs_accumulate(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_a
  * then mp_patch_E_mo_gyration, then mp_pabs_accumulate, bit for bit.  It requires every
  * list to be the film list: entry n of each ADE list and of the MO list is film cell n,
  * as FDTDState builds them from the material cells (Solver.jl:114-159); the caller
- * vouches for that (the lengths are checked).  ADE state must be stored in the compute
- * type; mo may be NULL (no gyration).  mo_ea / mo_eb are two of g's E volumes and
+ * vouches for that (the lengths are checked).  The three ADE lists share their storage:
+ * the compute type, or BF16 / F16 with scale (FP32 compute), decoded and encoded as
+ * mp_patch_E_dispersive does; mo may be NULL (no gyration).  mo_ea / mo_eb are two of g's E volumes and
  * mo_iea / mo_ieb their inv_eps volumes.  Replaces the launch sequence of
  * _ka_patch_E_dispersive! x3, _ka_patch_E_mo_gyration! and _ka_pabs_accumulate!
  * (Kernels.jl:595-646) when the lists coincide. */

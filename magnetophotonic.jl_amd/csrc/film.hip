@@ -9,19 +9,22 @@
 
 namespace mp {
 
-// P/J storage: CT, or BF16/F16 holding value*scale (FP32 arithmetic).
+// P/J storage: CT, or BF16/F16 holding value*scale (FP32 arithmetic).  enc/dec are the
+// conversions of one value; dec(enc(v)) is the value a later load of the stored v gives.
 template <typename C, typename T> struct StateIO {
-    static __device__ __forceinline__ C load(const T *p, int64_t o, float) { return (C)p[o]; }
-    static __device__ __forceinline__ void store(T *p, int64_t o, C v, float) { p[o] = (T)v; }
+    static __device__ __forceinline__ T enc(C v, float) { return (T)v; }
+    static __device__ __forceinline__ C dec(T q, float) { return (C)q; }
+    static __device__ __forceinline__ C load(const T *p, int64_t o, float isc) { return dec(p[o], isc); }
+    static __device__ __forceinline__ void store(T *p, int64_t o, C v, float sc) { p[o] = enc(v, sc); }
 };
-template <> struct StateIO<float, bf16> {
-    static __device__ __forceinline__ float load(const bf16 *p, int64_t o, float isc) { return ld<float>(p[o]) * isc; }
-    static __device__ __forceinline__ void store(bf16 *p, int64_t o, float v, float sc) { p[o] = st<bf16>(v * sc); }
+template <typename T> struct StateIOReduced {
+    static __device__ __forceinline__ T enc(float v, float sc) { return st<T>(v * sc); }
+    static __device__ __forceinline__ float dec(T q, float isc) { return ld<float>(q) * isc; }
+    static __device__ __forceinline__ float load(const T *p, int64_t o, float isc) { return dec(p[o], isc); }
+    static __device__ __forceinline__ void store(T *p, int64_t o, float v, float sc) { p[o] = enc(v, sc); }
 };
-template <> struct StateIO<float, f16s> {
-    static __device__ __forceinline__ float load(const f16s *p, int64_t o, float isc) { return ld<float>(p[o]) * isc; }
-    static __device__ __forceinline__ void store(f16s *p, int64_t o, float v, float sc) { p[o] = st<f16s>(v * sc); }
-};
+template <> struct StateIO<float, bf16> : StateIOReduced<bf16> {};
+template <> struct StateIO<float, f16s> : StateIOReduced<f16s> {};
 
 constexpr int MAXP = 8;
 
@@ -212,9 +215,11 @@ static int launch_pabs(mp_film *f, const mp_yee *g, const mp_ade *ax, const mp_a
 // mo_gyration_kernel and pabs_kernel, in that order, on its own values: the result is that
 // of the separate kernels bit for bit.  E is read and written once per cell instead of by
 // every kernel, J stays in registers for the absorbed power, and no position map is read.
-template <typename C> struct AdeView {
-    C *P, *J, *E_old;
+template <typename C, typename T> struct AdeView {
+    T *P, *J;
+    C *E_old;
     const C *fill, *inv_eps;
+    float sc, isc, jsc, ijsc;   // P and J storage scales and their inverses (1 for CT storage)
 };
 template <typename C> struct MoView {
     C *Pa, *Ja, *Pb, *Jb, *Eoa, *Eob;
@@ -222,9 +227,10 @@ template <typename C> struct MoView {
 };
 
 // ade_patch_kernel on one entry whose E is held in `Ecell` (storage type S).  jsum is the sum
-// over poles of J as stored after the call, in the order pabs_kernel adds it.
-template <typename S, typename C>
-__device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C> &a, const C *__restrict__ poles, int np,
+// over poles of J as stored after the call (decoded as pabs_kernel loads it), in the order
+// pabs_kernel adds it.
+template <typename S, typename C, typename T>
+__device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C, T> &a, const C *__restrict__ poles, int np,
                                           int64_t n, int64_t N, C inv_alpha, double &jsum) {
     const C Eprev = a.E_old[n];
     const C f = a.fill[n];
@@ -234,8 +240,8 @@ __device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C> &a, const C
 #pragma unroll
     for (int p = 0; p < MAXP; ++p) {
         if (p >= np) break;
-        Pold[p] = a.P[n + p * N];
-        Jold[p] = a.J[n + p * N];
+        Pold[p] = StateIO<C, T>::load(a.P, n + p * N, a.isc);
+        Jold[p] = StateIO<C, T>::load(a.J, n + p * N, a.ijsc);
         const C beta = poles[3 * p + 2] * f;
         const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
         sum_psi += psi;
@@ -255,9 +261,10 @@ __device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C> &a, const C
             const C psi = fmaT<C>(poles[3 * p], Pold[p], fmaT<C>(poles[3 * p + 1], Jold[p], beta * Eprev));
             const C Pnew = psi + beta * Efinal;
             const C Jnew = (Pnew - Pold[p]) * inv_alpha - Jold[p];
-            a.P[n + p * N] = Pnew;
-            a.J[n + p * N] = Jnew;
-            jsum += (double)Jnew;
+            StateIO<C, T>::store(a.P, n + p * N, Pnew, a.sc);
+            const T jq = StateIO<C, T>::enc(Jnew, a.jsc);
+            a.J[n + p * N] = jq;
+            jsum += (double)StateIO<C, T>::dec(jq, a.ijsc);
         }
     } else {
 #pragma unroll
@@ -268,10 +275,10 @@ __device__ __forceinline__ void ade_entry(S &Ecell, const AdeView<C> &a, const C
     }
 }
 
-template <typename S, typename C, bool HAS_MO, bool ADE_WORK>
+template <typename S, typename C, typename T, bool HAS_MO, bool ADE_WORK>
 __global__ __launch_bounds__(256) void film_epilogue_kernel(
     S *__restrict__ Ex, S *__restrict__ Ey, S *__restrict__ Ez, const int32_t *__restrict__ cells,
-    AdeView<C> ax, AdeView<C> ay, AdeView<C> az, const C *__restrict__ poles, int np, C inv_alpha,
+    AdeView<C, T> ax, AdeView<C, T> ay, AdeView<C, T> az, const C *__restrict__ poles, int np, C inv_alpha,
     MoView<C> mo, const C *__restrict__ mo_poles, int mo_np, int mo_a, int mo_b,
     const S *__restrict__ iea, const S *__restrict__ ieb, C Q_TM, C Q_RE, C mo_inv_alpha,
     double *__restrict__ U_abs, double *__restrict__ window, double *__restrict__ peak, double pabs_factor,
@@ -281,9 +288,9 @@ __global__ __launch_bounds__(256) void film_epilogue_kernel(
     const int64_t idx = cells[n];
     S e[3] = {Ex[idx], Ey[idx], Ez[idx]};
     double js[3];
-    ade_entry<S, C>(e[0], ax, poles, np, n, N, inv_alpha, js[0]);
-    ade_entry<S, C>(e[1], ay, poles, np, n, N, inv_alpha, js[1]);
-    ade_entry<S, C>(e[2], az, poles, np, n, N, inv_alpha, js[2]);
+    ade_entry<S, C, T>(e[0], ax, poles, np, n, N, inv_alpha, js[0]);
+    ade_entry<S, C, T>(e[1], ay, poles, np, n, N, inv_alpha, js[1]);
+    ade_entry<S, C, T>(e[2], az, poles, np, n, N, inv_alpha, js[2]);
     if (HAS_MO) {
         // mo_gyration_kernel on entry n (its film position is n)
         const C f = mo.fill[n];
@@ -348,15 +355,20 @@ __global__ __launch_bounds__(256) void film_epilogue_kernel(
     if (pabs > peak[n]) peak[n] = pabs;
 }
 
-template <typename S, typename C>
+template <typename S, typename C, typename T>
 static int launch_epilogue(const mp_yee *g, mp_ade *const ade[3], const mp_mo *mo, int mo_a, int mo_b,
                            const void *iea, const void *ieb, mp_film *f, int has_ade_work, double pabs_factor,
                            double Q_TM, double Q_RE, double dt) {
     const int64_t N = f->n;
-    AdeView<C> v[3];
-    for (int c = 0; c < 3; ++c)
-        v[c] = AdeView<C>{(C *)ade[c]->P, (C *)ade[c]->J, (C *)ade[c]->E_old, (const C *)ade[c]->fill,
-                          (const C *)ade[c]->inv_eps};
+    AdeView<C, T> v[3];
+    for (int c = 0; c < 3; ++c) {
+        // the scales of launch_ade (P, J) and launch_pabs (J), formed the same way
+        const double sc = state_scale(ade[c]);
+        const double jsc = reduced(ade[c]->state_dtype) ? sc * dt / 2.0 : 1.0;
+        v[c] = AdeView<C, T>{(T *)ade[c]->P, (T *)ade[c]->J, (C *)ade[c]->E_old, (const C *)ade[c]->fill,
+                             (const C *)ade[c]->inv_eps, (float)sc, (float)(1.0 / sc), (float)jsc,
+                             (float)(1.0 / jsc)};
+    }
     MoView<C> m{};
     if (mo)
         m = MoView<C>{(C *)mo->Pa_from_b, (C *)mo->Ja_from_b, (C *)mo->Pb_from_a, (C *)mo->Jb_from_a,
@@ -372,11 +384,11 @@ static int launch_epilogue(const mp_yee *g, mp_ade *const ade[3], const mp_mo *m
                 pabs_factor, dt, N
     const unsigned grid = cdiv(N, 256);
     if (mo) {
-        if (has_ade_work) film_epilogue_kernel<S, C, true, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
-        else film_epilogue_kernel<S, C, true, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        if (has_ade_work) film_epilogue_kernel<S, C, T, true, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        else film_epilogue_kernel<S, C, T, true, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
     } else {
-        if (has_ade_work) film_epilogue_kernel<S, C, false, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
-        else film_epilogue_kernel<S, C, false, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        if (has_ade_work) film_epilogue_kernel<S, C, T, false, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        else film_epilogue_kernel<S, C, T, false, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
     }
 #undef EP_ARGS
     MP_LAUNCH_CHECK("film_epilogue_kernel");
@@ -491,8 +503,10 @@ MP_API int mp_film_epilogue(const mp_yee *g, mp_ade *ax, mp_ade *ay, mp_ade *az,
                                        (long long)a->n, (long long)N);
         if (a->npoles != ax->npoles) return fail_arg("mp_film_epilogue: ADE lists disagree on npoles");
         if (a->npoles < 1 || a->npoles > MAXP) return fail_arg("mp_film_epilogue: 1..%d poles", MAXP);
-        if (a->dtype != ax->dtype || a->state_dtype != a->dtype)
-            return fail_arg("mp_film_epilogue: ADE state must be stored in the compute type");
+        if (a->dtype != ax->dtype || a->state_dtype != ax->state_dtype || state_scale(a) != state_scale(ax))
+            return fail_arg("mp_film_epilogue: the three ADE lists must share compute type, storage and scale");
+        if (a->state_dtype != a->dtype && !(a->dtype == MP_F32 && reduced(a->state_dtype)))
+            return fail_arg("mp_film_epilogue: ADE state stored in the compute type, or BF16/F16 with FP32");
         if (!a->P || !a->J || !a->E_old || !a->fill || !a->inv_eps || !a->poles)
             return fail_arg("mp_film_epilogue: null ADE array");
     }
@@ -511,15 +525,16 @@ MP_API int mp_film_epilogue(const mp_yee *g, mp_ade *ax, mp_ade *ay, mp_ade *az,
     } else {
         mo = nullptr;
     }
-    const int32_t fd = g->field_dtype, ct = ax->dtype;
-    if (fd == MP_F32 && ct == MP_F32)
-        return launch_epilogue<float, float>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
-                                             Q_TM, Q_RE, dt);
-    if (fd == MP_F64 && ct == MP_F64)
-        return launch_epilogue<double, double>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
-                                               Q_TM, Q_RE, dt);
-    if (fd == MP_F32 && ct == MP_F64)
-        return launch_epilogue<float, double>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor,
-                                              Q_TM, Q_RE, dt);
+    const int32_t fd = g->field_dtype, ct = ax->dtype, sd = ax->state_dtype;
+#define EP_CALL(S, C, T) \
+    launch_epilogue<S, C, T>(g, ade, mo, mo_a, mo_b, mo_iea, mo_ieb, f, has_ade_work, pabs_factor, Q_TM, Q_RE, dt)
+    if (fd == MP_F32 && ct == MP_F32) {
+        if (sd == MP_BF16) return EP_CALL(float, float, bf16);
+        if (sd == MP_F16) return EP_CALL(float, float, f16s);
+        return EP_CALL(float, float, float);
+    }
+    if (fd == MP_F64 && ct == MP_F64) return EP_CALL(double, double, double);
+    if (fd == MP_F32 && ct == MP_F64) return EP_CALL(float, double, double);
+#undef EP_CALL
     return fail_arg("mp_film_epilogue: unsupported dtypes field=%d ct=%d", fd, ct);
 }

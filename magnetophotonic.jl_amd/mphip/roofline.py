@@ -63,7 +63,7 @@ def kernel_bytes(spec, eps_index: bool = False, fused_epilogue: bool = False) ->
         out["pabs"] = nf * (4 + 3 * fs + work + 6 * 8)
         out["multiphysics"] = nf * (20 * cs + 2 * 8)
         if fused_epilogue and spec.ade is not None:
-            per = 4 + 6 * fs + 3 * (4 * cs + 4 * np_ * cs) + 6 * 8   # cell, E r/w, 3 ADE entries, U/window/peak
+            per = 4 + 6 * fs + 3 * (4 * cs + 4 * np_ * ss) + 6 * 8   # cell, E r/w, 3 ADE entries, U/window/peak
             if spec.mo is not None:
                 npm = len(spec.mo_poles) // 3
                 per += 3 * cs + 4 * cs + 8 * npm * cs + 2 * fs       # fill, m x2, E_old r/w x2, P/J, inv_eps x2

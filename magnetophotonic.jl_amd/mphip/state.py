@@ -355,13 +355,15 @@ class FDTDState:
 
     def _lists_coincide(self) -> bool:
         """Every ADE list and the MO list is the film list (entry n is film cell n) and the
-        ADE state is stored in the compute type: the step may run its list work as one
-        mp_film_epilogue pass (the lists FDTDState builds from the material cells,
-        Solver.jl:114-159)."""
+        ADE state is stored in the compute type, or in BF16/F16 with FP32 compute: the step
+        may run its list work as one mp_film_epilogue pass (the lists FDTDState builds from
+        the material cells, Solver.jl:114-159)."""
         spec = self.spec
         if self.film is None or spec.ade is None or spec.n_film == 0:
             return False
-        if _PSI_CODE[spec.ade_state_dtype] != _DT_CODE[np.dtype(spec.compute_dtype)]:
+        ct = _DT_CODE[np.dtype(spec.compute_dtype)]
+        sd = _PSI_CODE[spec.ade_state_dtype]
+        if sd != ct and not (ct == abi.MP_F32 and sd in (abi.MP_BF16, abi.MP_F16)):
             return False
         ident = np.arange(spec.n_film, dtype=np.int64)
         for c in range(3):

@@ -336,3 +336,89 @@ def relax_comparison(sur: "Surrogate", ref_film: dict, fluence: float = 60.0, pu
     out["switched_ref"] = float(np.mean(np.asarray(ref_film["m_TM_x"]) < 0))
     st.free()
     return out
+
+
+# ---- the storage + surrogate forms on the accuracy case ----------------------------------------
+def pump_relax_rows(fluences, pump_steps: int = 1500, pump_every: int = 25, relax_steps: int = 600,
+                    relax_every: int = 5, stride: int = 4, relax_dt: float = 5e-15, relax_subcycles: int = 10):
+    """Training rows of both phases of the accuracy case at each fluence (plain FP32 runs):
+    every `pump_every`-th pump step and every `relax_every`-th relax step, every `stride`-th
+    film cell."""
+    from . import configs
+    Xs, Ts = [], []
+    for fl in fluences:
+        spec, _ = configs.coupled_pump(dtype="f32", steps=pump_steps, fluence_mj_cm2=float(fl), **ACCURACY_CASE)
+        st = FDTDState(spec)
+        X, T = collect_trajectory(st, pump_steps, pump_every, relax_steps, relax_dt, relax_subcycles, relax_every,
+                                  stride)
+        st.free()
+        Xs.append(X)
+        Ts.append(T)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def film_mean_dm(film: dict, ref_film: dict) -> float:
+    """Final film-mean |dm| against a reference film state, the larger of the two sublattices."""
+    return max(abs(float(np.asarray(film[k], np.float64).mean() - np.asarray(ref_film[k], np.float64).mean()))
+               for k in ("m_TM_x", "m_RE_x"))
+
+
+def accuracy_run(kind: Optional[str] = None, sur: Optional["Surrogate"] = None, tol: float = 1e-4,
+                 fluence: float = 60.0, pump_steps: int = 1500, relax_steps: int = 600, relax_dt: float = 5e-15,
+                 relax_subcycles: int = 10) -> dict:
+    """The accuracy case (pump + relax) in one form: CPML psi and ADE state stored as `kind`
+    ('bf16' / 'f16'; None keeps FP32) and the film update by `sur` in both phases (None: the
+    plain integrator).  Returns the final film state, the GPU wall time of the whole case and
+    the fraction of film-cell updates that fell back to the plain integrator."""
+    import time
+
+    from . import _lib, configs
+    kw = dict(psi_precision=kind, ade_state_precision=kind) if kind else {}
+    spec, _ = configs.coupled_pump(dtype="f32", steps=pump_steps, fluence_mj_cm2=fluence, **ACCURACY_CASE, **kw)
+    st = FDTDState(spec)
+    if sur is not None:
+        st.attach_surrogate(sur, tol)
+        sur.reset_fallbacks()
+    _lib.call("synchronize")
+    t0 = time.perf_counter()
+    st.run(pump_steps)
+    st.flush_multiphysics()
+    for _ in range(relax_steps):
+        st.relax_step(relax_dt, subcycles=relax_subcycles)
+    _lib.call("synchronize")
+    ms = 1e3 * (time.perf_counter() - t0)
+    updates = spec.n_film * (pump_steps // max(spec.multiphysics_every, 1) + relax_steps)
+    fb = sur.fallback_count() / updates if sur is not None else 0.0
+    film = st.film_state()
+    st.free()
+    return dict(film=film, ms=ms, fallback=fb)
+
+
+def accuracy_forms(kinds=("bf16", "f16"), tol: float = 1e-4, fluences=(35.0, 50.0, 70.0, 85.0)) -> dict:
+    """Film-mean |dm| of the accuracy case at 60 mJ/cm^2 against the same path in FP64 on
+    the GPU, for plain FP32, for the surrogate with FP32 storage and for the surrogate with
+    each reduced storage kind.  One network serves both phases; it is trained on the GPU on
+    rows of both phases of runs at the training fluences (60 is held out)."""
+    import time
+    t0 = time.perf_counter()
+    X, T = pump_relax_rows(fluences)
+    sur = train_relax_surrogate(X, T)
+    rows = int(X.shape[0])
+    del X, T
+    ref = pump_relax_film("f64")
+    out = {}
+
+    def entry(r):
+        return dict(film_mean_dm=film_mean_dm(r["film"], ref), case_ms=round(r["ms"], 1),
+                    fallback=round(r["fallback"], 4), switched=float(np.mean(np.asarray(r["film"]["m_TM_x"]) < 0)))
+
+    out["plain"] = entry(accuracy_run(None))
+    out["surrogate"] = entry(accuracy_run(None, sur, tol))
+    for k in kinds:
+        out[f"surrogate+{k}"] = entry(accuracy_run(k, sur, tol))
+    out["switched_ref"] = float(np.mean(np.asarray(ref["m_TM_x"]) < 0))
+    out["train_rows"] = rows
+    out["tol"] = tol
+    out["study_s"] = round(time.perf_counter() - t0, 1)
+    sur.free()
+    return out

@@ -84,6 +84,37 @@ def test_reduced_storage_with_gyration():
     gpu.free()
 
 
+@pytest.mark.parametrize("kind", ["bf16", "f16"])
+def test_reduced_storage_fused_pass_matches_five_launches(kind):
+    """With BF16 / binary16 ADE state the step's list work still runs as one fused pass; it
+    must give what the five separate launches give, bit for bit (gyration on)."""
+    from mphip import configs
+    from mphip.state import FDTDState
+    spec, _ = configs.coupled_pump(nx=96, ny=68, nz=64, steps=150, fluence_mj_cm2=12.0, npml=6,
+                                   src_z=40e-9, n_planes=0, psi_precision=kind, ade_state_precision=kind)
+    print(f"case reduced storage {kind} fused vs five launches 96x68x64", flush=True)
+    fused, sep = FDTDState(spec), FDTDState(spec)
+    assert fused.fused_epilogue
+    sep.fused_epilogue = False
+    fused.run(150)
+    sep.run(150)
+    fused.synchronize()
+    sep.synchronize()
+    for name in FIELDS:
+        assert np.array_equal(fused.field(name), sep.field(name)), name
+    for c in range(3):
+        g, r = fused.ade_state(c), sep.ade_state(c)
+        for k in ("P", "J", "E_old"):
+            assert np.array_equal(g[k], r[k]), (c, k)
+        assert np.any(r["J"])
+    fa, fb = fused.film_state(), sep.film_state()
+    assert fb["Te"].max() > 300.5
+    for k in fb:
+        assert np.array_equal(fa[k], fb[k]), k
+    fused.free()
+    sep.free()
+
+
 def _fp64_reference():
     if "ref" not in _REF:
         import driver
