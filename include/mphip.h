@@ -570,6 +570,16 @@ int mp_plane_gather(const void *field, int32_t dtype, int64_t nx, int64_t ny, in
 int mp_probe_trace_samples(const mp_yee *g, int64_t i_pre, int64_t i_post, const double *mode_w,
                            const double *area, double *work);
 
+/* One probe movie frame, staged on the device (_record_probe_frame!,
+ * src/sim/ProbeReadout.jl:193-215, which downloads the six planes with plane_to_host and
+ * takes Float32.(abs.(...)) on the host): |Ey| and |Ez| as float on the z-plane kc (0-based)
+ * into xy_* (nx*ny, x fastest, the reference's (Nx, Ny) slice) and on the x-planes i_pre,
+ * i_post into yz_* (ny*nz, y fastest, (Ny, Nz)).  The outputs are device arrays, typically
+ * frame f of a frame-last (d1, d2, frames) cube, the layout _stack_frames builds
+ * (ProbeReadout.jl:172-181). */
+int mp_probe_frames(const mp_yee *g, int64_t kc, int64_t i_pre, int64_t i_post, float *xy_ey, float *xy_ez,
+                    float *yz_ey_pre, float *yz_ez_pre, float *yz_ey_post, float *yz_ez_post);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,7 @@
 // FilmAverage / SwitchedFraction (Monitors.jl:487-578); NaNGuard (Monitors.jl:580-606).
 // Reductions are two-pass (fixed per-block partials, then one ordered final sum) so
 // they are deterministic run to run.
+#include <algorithm>
 #include <mutex>
 
 #include "common.h"
@@ -996,6 +997,30 @@ __global__ __launch_bounds__(256) void trace_samples_kernel(const S *Ey, const S
     work[c + 3 * np] = (double)Ez[o2] * w;
 }
 
+// One probe movie frame (_record_probe_frame!, src/sim/ProbeReadout.jl:193-215): |Ey| and
+// |Ez| rounded to float on the z-plane kc (nx, ny) and on the x-planes ipre, ipost (ny, nz),
+// first index fastest.  blockIdx.y picks the plane (0: z, 1: pre, 2: post).
+template <typename S>
+__global__ __launch_bounds__(256) void probe_frames_kernel(const S *Ey, const S *Ez, int64_t nx, int64_t ny,
+                                                           int64_t nz, int64_t kc, int64_t ipre, int64_t ipost,
+                                                           float *xy_ey, float *xy_ez, float *pre_ey, float *pre_ez,
+                                                           float *post_ey, float *post_ez) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.y == 0) {
+        if (c >= nx * ny) return;
+        const int64_t o = c + nx * ny * kc;
+        xy_ey[c] = (float)__builtin_fabs(Ey[o]);
+        xy_ez[c] = (float)__builtin_fabs(Ez[o]);
+    } else {
+        if (c >= ny * nz) return;
+        const int64_t o = (blockIdx.y == 1 ? ipre : ipost) + nx * c;
+        float *dy = blockIdx.y == 1 ? pre_ey : post_ey;
+        float *dz = blockIdx.y == 1 ? pre_ez : post_ez;
+        dy[c] = (float)__builtin_fabs(Ey[o]);
+        dz[c] = (float)__builtin_fabs(Ez[o]);
+    }
+}
+
 }  // namespace mp
 
 MP_API int mp_reduce(const void *x, const void *y, const void *z, int32_t dtype, int64_t n, int32_t op, double *out) {
@@ -1070,5 +1095,30 @@ MP_API int mp_probe_trace_samples(const mp_yee *g, int64_t i_pre, int64_t i_post
     else
         return fail_arg("mp_probe_trace_samples: unsupported field dtype");
     MP_LAUNCH_CHECK("trace_samples_kernel");
+    return MP_OK;
+}
+
+MP_API int mp_probe_frames(const mp_yee *g, int64_t kc, int64_t i_pre, int64_t i_post, float *xy_ey, float *xy_ez,
+                           float *yz_ey_pre, float *yz_ez_pre, float *yz_ey_post, float *yz_ez_post) {
+    if (!g || !xy_ey || !xy_ez || !yz_ey_pre || !yz_ez_pre || !yz_ey_post || !yz_ez_post)
+        return fail_arg("mp_probe_frames: null argument");
+    if (kc < 0 || kc >= g->nz) return fail_arg("mp_probe_frames: z slice index out of range");
+    if (i_pre < 0 || i_pre >= g->nx || i_post < 0 || i_post >= g->nx)
+        return fail_arg("mp_probe_frames: plane index out of range");
+    const int64_t n = std::max(g->nx * g->ny, g->ny * g->nz);
+    if (n == 0) return MP_OK;
+    const dim3 grid(cdiv(n, 256), 3);
+    KernelTimer t("probe_frames");
+    if (g->field_dtype == MP_F32)
+        probe_frames_kernel<float><<<grid, 256, 0, stream()>>>((const float *)g->ey, (const float *)g->ez, g->nx, g->ny,
+                                                               g->nz, kc, i_pre, i_post, xy_ey, xy_ez, yz_ey_pre,
+                                                               yz_ez_pre, yz_ey_post, yz_ez_post);
+    else if (g->field_dtype == MP_F64)
+        probe_frames_kernel<double><<<grid, 256, 0, stream()>>>((const double *)g->ey, (const double *)g->ez, g->nx,
+                                                                g->ny, g->nz, kc, i_pre, i_post, xy_ey, xy_ez,
+                                                                yz_ey_pre, yz_ez_pre, yz_ey_post, yz_ez_post);
+    else
+        return fail_arg("mp_probe_frames: unsupported field dtype");
+    MP_LAUNCH_CHECK("probe_frames_kernel");
     return MP_OK;
 }

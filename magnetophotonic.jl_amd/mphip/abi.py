@@ -129,6 +129,7 @@ HOOK_SIGNATURES = {
     "sum_blocks": (C.c_int, [vp, i64, i64, vp]),
     "plane_gather": (C.c_int, [vp, i32, i64, i64, i64, i32, i64, vp]),
     "probe_trace_samples": (C.c_int, [P(Yee), i64, i64, vp, vp, vp]),
+    "probe_frames": (C.c_int, [P(Yee), i64, i64, i64, vp, vp, vp, vp, vp, vp]),
 }
 
 # No-allocation variants of entry points (fixed library scratch); no oracle counterpart.

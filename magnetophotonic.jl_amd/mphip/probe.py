@@ -14,8 +14,11 @@ is `shot_from_reductions`, which needs no device.
 src/sim/Monitors.jl (:386-403, :461-485, :608-635) with the plane sums on the device
 (mp_plane_overlap, mp_flux_plane) and their series kept in device slots.
 
-Movie frames (ProbeReadout.jl:186-255) are host copies of field planes for the HDF5
-writer and are not recorded here: `frame_target` must be 0.
+Movie frames (_probe_frame_due / _record_probe_frame!, ProbeReadout.jl:183-215) are staged
+on the device: at each due step one mp_probe_frames launch writes |Ey| and |Ez| (float) of
+the z slice and the pre and post x-planes into frame-last device cubes (the layout of
+_stack_frames, :172-181); full chunks of frames are downloaded, and probe_shot returns the
+six cubes with the frame indices and times (:465-475).
 """
 from __future__ import annotations
 
@@ -242,6 +245,58 @@ class _Slots:
             self.buf = None
 
 
+class _FrameStore:
+    """Probe movie frames on the device: six frame-last float32 cubes, (nx, ny) slices of
+    Ey, Ez at z = kc and (ny, nz) slices at the pre and post x-planes, in chunks of `chunk`
+    frames; a full chunk is downloaded before the next frame is staged."""
+
+    KEYS = ("Ey_xy", "Ez_xy", "Ey_yz_pre", "Ez_yz_pre", "Ey_yz_post", "Ez_yz_post")
+
+    def __init__(self, nx: int, ny: int, nz: int, chunk: int = 32):
+        self.dims = {k: ((nx, ny) if k.endswith("_xy") else (ny, nz)) for k in self.KEYS}
+        self.chunk = int(chunk)
+        self.count = 0
+        self.host = {k: [] for k in self.KEYS}
+        self.bufs = None
+
+    def _download(self, k, frames):
+        d1, d2 = self.dims[k]
+        a = self.bufs[k].download(np.float32, d1 * d2 * frames)
+        return a.reshape((d1, d2, frames), order="F")
+
+    def next_ptrs(self):
+        """Device pointers of the next frame's six planes (mp_probe_frames order)."""
+        from .state import DeviceBuffer
+        if self.bufs is None:
+            self.bufs = {k: DeviceBuffer.zeros(4 * d1 * d2 * self.chunk) for k, (d1, d2) in self.dims.items()}
+            self.count = 0
+        if self.count == self.chunk:
+            for k in self.KEYS:
+                self.host[k].append(self._download(k, self.chunk))
+            self.count = 0
+        ptrs = []
+        for k in self.KEYS:
+            d1, d2 = self.dims[k]
+            ptrs.append(C.c_void_p(self.bufs[k].ptr + 4 * d1 * d2 * self.count))
+        self.count += 1
+        return ptrs
+
+    def frames(self) -> dict:
+        out = {}
+        for k in self.KEYS:
+            parts = list(self.host[k])
+            if self.bufs is not None and self.count:
+                parts.append(self._download(k, self.count))
+            out[k] = np.concatenate(parts, axis=2) if parts else np.zeros((0, 0, 0), np.float32)
+        return out
+
+    def free(self):
+        if self.bufs is not None:
+            for b in self.bufs.values():
+                b.free()
+            self.bufs = None
+
+
 def _plane_index(state, axis: int, position) -> int:
     from .sources import source_index
     g = state.spec.grid
@@ -252,11 +307,19 @@ class ProbeReadout:
     """Gold-standard probe read-out on the device (ProbeReadout.jl:11-478)."""
 
     def __init__(self, pre, post, mode=None, lambda0: float = 532e-9, omega_bins=None, spectrum_bins: int = 5,
-                 trace_stride: int = 10, frame_target: int = 0, probe_amplitude: float = 1.0,
-                 probe_tau: float = 24e-15, final_step: int = 0):
-        if int(frame_target) != 0:
-            raise ValueError("movie frames are not recorded by the device read-out: frame_target must be 0")
+                 trace_stride: int = 10, frame_target: int = 0, frame_skip: int = 0, store_frames: bool = True,
+                 probe_amplitude: float = 1.0, probe_tau: float = 24e-15, frame_indices=(),
+                 slice_z_position: float = math.nan, final_step: int = 0):
         self.x_pre, self.x_post = pre, post
+        # the frame plan and its stride (ProbeReadout.jl:98-102)
+        plan = [int(v) for v in frame_indices]
+        skip = int(frame_skip)
+        if skip <= 0 and len(plan) > 1:
+            skip = max(1, plan[1] - plan[0])
+        self.frame_target, self.frame_skip, self.store_frames = int(frame_target), skip, bool(store_frames)
+        self.frame_plan, self.frame_ptr = plan, 0
+        self.slice_z_position = float(slice_z_position)
+        self.frame_indices, self.frame_times = [], []
         self.mode = mode
         self.lambda0 = float(lambda0)
         if omega_bins is None:
@@ -291,6 +354,10 @@ class ProbeReadout:
         self.mode_w_dev = DeviceBuffer.from_array(self.mode_w)
         self.red_dev = DeviceBuffer.zeros(8 * (4 * len(self.omega_bins) + 10))
         self.traces = _Slots(4)
+        # the xy movie slice: the domain's middle z plane unless a position is given (:197-198)
+        self.kc = (-(-g.z.n // 2) - 1) if math.isnan(self.slice_z_position) else _plane_index(
+            state, 2, self.slice_z_position)
+        self.frames_store = _FrameStore(g.x.n, ny, nz)
         self.state = state
         self.initialized = True
 
@@ -308,6 +375,30 @@ class ProbeReadout:
         """n == 1, every trace_stride-th step, and the final step (ProbeReadout.jl:111-112)."""
         return n == 1 or n % self.trace_stride == 0 or (self.final_step > 0 and n == self.final_step)
 
+    def frame_due(self, n: int) -> bool:
+        """_probe_frame_due (ProbeReadout.jl:183-191): the next planned step, else every
+        frame_skip-th step (or every ceil(n / frame_target)-th) until frame_target frames."""
+        if self.frame_target <= 0:
+            return False
+        if self.frame_plan:
+            return self.frame_ptr < len(self.frame_plan) and n == self.frame_plan[self.frame_ptr]
+        skip = self.frame_skip if self.frame_skip > 0 else max(1, -(-max(n, 1) // self.frame_target))
+        return n % skip == 0 and len(self.frame_indices) < self.frame_target
+
+    def _record_frame(self, state):
+        """_record_probe_frame! (ProbeReadout.jl:193-215) with the planes staged on the device."""
+        if not self.frame_due(state.n):
+            return
+        if self.frame_plan:
+            self.frame_ptr += 1
+        self.frame_indices.append(int(state.n))
+        self.frame_times.append(float(state.t))
+        if not self.store_frames:
+            return
+        from . import _lib
+        _lib.call("probe_frames", C.byref(state.yee), self.kc, self.i_pre, self.i_post,
+                  *self.frames_store.next_ptrs())
+
     def _record_trace(self, state):
         from . import _lib
         _lib.call("probe_trace_plane", C.byref(state.yee), 0, self.i_pre, self.i_post, self.mode_w_dev.vp,
@@ -320,22 +411,24 @@ class ProbeReadout:
         self.planes.record(state)
         if self.trace_due(state.n):
             self._record_trace(state)
+        self._record_frame(state)
         return self
 
     def run(self, state, steps: int) -> "ProbeReadout":
         """`steps` steps of `state` with the read-out recorded after each: the DFT inside the
-        step driver, the traces between driver calls at the due steps."""
+        step driver, the traces and movie frames between driver calls at the due steps."""
         self._ensure(state)
         state.attach_dft(self.planes)
         try:
             end = state.n + int(steps)
             while state.n < end:
                 n = state.n + 1
-                while n < end and not self.trace_due(n):
+                while n < end and not (self.trace_due(n) or self.frame_due(n)):
                     n += 1
                 state.run(n - state.n)
                 if self.trace_due(state.n):
                     self._record_trace(state)
+                self._record_frame(state)
         finally:
             state.detach_dft()
         return self
@@ -361,7 +454,7 @@ class ProbeReadout:
     def probe_shot(self, state_label: str = "probe", is_reference: bool = False, incident_energy=None,
                    incident_sx=None, normalization_method: str = "gold_standard_bare_waveguide_reference_netflux",
                    dt: float = math.nan, steps_probe: int = 0, absorbed_energy=None) -> dict:
-        """probe_shot (ProbeReadout.jl:362-478) without the movie frames."""
+        """probe_shot (ProbeReadout.jl:362-478)."""
         nb = len(self.omega_bins)
         red = self.reductions()
         out = shot_from_reductions(red, nb, self.mode_norm if self.initialized else 1.0,
@@ -383,6 +476,11 @@ class ProbeReadout:
             normalization_method=str(normalization_method), dt=float(dt), lambda_nm=self.lambda0 * 1e9,
             probe_amplitude_V_m=self.probe_amplitude, probe_tau_fs=self.probe_tau * 1e15,
             probe_trace_stride=self.trace_stride, steps_probe=int(steps_probe),
+            probe_frame_target=self.frame_target, probe_frame_skip=self.frame_skip,
+            probe_frame_count=len(self.frame_indices), probe_frame_indices=list(self.frame_indices),
+            probe_frame_times_s=list(self.frame_times),
+            frames=(self.frames_store.frames() if self.initialized
+                    else {k: np.zeros((0, 0, 0), np.float32) for k in _FrameStore.KEYS}),
         )
         return out
 
@@ -392,6 +490,7 @@ class ProbeReadout:
             for b in (self.area_dev, self.mode_w_dev, self.red_dev):
                 b.free()
             self.traces.free()
+            self.frames_store.free()
             self.initialized = False
 
 

@@ -268,3 +268,59 @@ def test_polarimetry_and_transmission_match_oracle():
     for m in mons:
         m.free()
     gpu.free()
+
+
+@pytest.mark.parametrize("how", ["run", "record"])
+def test_probe_frames_match_oracle_slices(how):
+    """Movie frames staged on the device (mp_probe_frames) against Float32.(abs.(...)) of the
+    oracle's field planes at the same steps (_record_probe_frame!, ProbeReadout.jl:193-215):
+    the z slice and the pre and post x-planes, stacked frame-last.  `run` takes frames every
+    5th step across two driver batches, on the domain's middle z plane; `record` takes one
+    every step up to 40 (frame_skip 0) on a given z position, past one 32-frame chunk."""
+    import driver
+    from mphip.probe import ProbeReadout
+    from mphip.state import FDTDState
+    spec = _x_probe_spec()
+    steps = 70
+    print(f"case probe frames {how} {spec.shape}", flush=True)
+    gpu = FDTDState(spec)
+    ref = driver.OracleState(spec)
+    kw = dict(pre=14, post=34, lambda0=532e-9, trace_stride=10)
+    if how == "run":
+        ro = ProbeReadout(frame_target=12, frame_skip=5, **kw)
+        ro.run(gpu, 33)
+        ro.run(gpu, steps - 33)
+        sched, kc = list(range(5, 61, 5)), (spec.shape[2] + 1) // 2 - 1
+    else:
+        ro = ProbeReadout(frame_target=40, slice_z_position=float(spec.grid.z.centers[3]), **kw)
+        for _ in range(steps):
+            gpu.step()
+            ro.record(gpu)
+        sched, kc = list(range(1, 41)), 3
+    assert ro.frame_indices == sched and ro.kc == kc
+    want = {k: [] for k in ("Ey_xy", "Ez_xy", "Ey_yz_pre", "Ez_yz_pre", "Ey_yz_post", "Ez_yz_post")}
+    times = []
+    for n in range(1, steps + 1):
+        ref.step()
+        if n in sched:
+            ey, ez = ref.field("ey"), ref.field("ez")
+            for k, f in (("Ey", ey), ("Ez", ez)):
+                want[k + "_xy"].append(np.abs(f[:, :, kc]).astype(np.float32))
+                want[k + "_yz_pre"].append(np.abs(f[14, :, :]).astype(np.float32))
+                want[k + "_yz_post"].append(np.abs(f[34, :, :]).astype(np.float32))
+            times.append(ref.t)
+    for k in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        assert np.array_equal(gpu.field(k), ref.field(k)), k
+    shot = ro.probe_shot(state_label=how, dt=spec.dt, steps_probe=steps)
+    assert shot["probe_frame_count"] == len(sched) and shot["probe_frame_indices"] == sched
+    np.testing.assert_allclose(shot["probe_frame_times_s"], times, rtol=1e-14)
+    nx, ny, nz = spec.shape
+    for k, frames in want.items():
+        cube = shot["frames"][k]
+        assert cube.dtype == np.float32
+        assert cube.shape == ((nx, ny) if k.endswith("_xy") else (ny, nz)) + (len(sched),), (k, cube.shape)
+        w = np.stack(frames, axis=2)
+        assert np.array_equal(cube, w), (k, int(np.count_nonzero(cube != w)))
+        assert np.any(w > 0), k
+    ro.free()
+    gpu.free()

@@ -103,9 +103,47 @@ def test_default_bins_and_centre():
     assert ro.center_bin == 1          # first of the two nearest, as argmin
     assert ProbeReadout(pre=1e-7, post=4e-7, omega_bins=[1.0, 2.0, w0]).center_bin == 2
     with pytest.raises(ValueError):
-        ProbeReadout(pre=1e-7, post=4e-7, frame_target=10)
-    with pytest.raises(ValueError):
         default_bins(532e-9, 1)
+
+
+def _frame_schedule(ro, first, last):
+    """The steps at which the read-out takes a movie frame, with store_frames off (no device)."""
+    class _State:
+        pass
+    st = _State()
+    for n in range(first, last + 1):
+        st.n, st.t = n, n * 1e-15
+        ro._record_frame(st)
+    return list(ro.frame_indices)
+
+
+def test_probe_frame_schedule():
+    """_probe_frame_due (ProbeReadout.jl:183-191), schedules worked out by hand from its rules:
+    a plan is followed entry by entry; else every frame_skip-th step, or with frame_skip 0
+    every ceil(n / frame_target)-th step, until frame_target frames are taken."""
+    from mphip.probe import ProbeReadout
+
+    def ro(**kw):
+        return ProbeReadout(pre=1e-7, post=4e-7, store_frames=False, **kw)
+
+    assert _frame_schedule(ro(), 1, 50) == []                                   # frame_target 0: none
+    assert _frame_schedule(ro(frame_target=3), 1, 50) == [1, 2, 3]              # ceil(n/3) = 1 for n <= 3
+    assert _frame_schedule(ro(frame_target=4, frame_skip=5), 1, 50) == [5, 10, 15, 20]
+    # from step 101 with frame_target 4: ceil(104/4) = 26 divides 104, ceil(108/4) = 27 divides
+    # 108, then 112 (28) and 116 (29); 101-103, 105-107, ... are not multiples of theirs
+    assert _frame_schedule(ro(frame_target=4), 101, 200) == [104, 108, 112, 116]
+    r = ro(frame_target=3, frame_indices=[3, 7, 11])
+    assert r.frame_skip == 4                                                     # plan[2] - plan[1]
+    assert _frame_schedule(r, 1, 50) == [3, 7, 11] and r.frame_ptr == 3
+    assert _frame_schedule(ro(frame_target=0, frame_indices=[3, 7]), 1, 50) == []
+    r = ro(frame_target=5, frame_indices=[2, 4])
+    assert _frame_schedule(r, 1, 50) == [2, 4]                                   # the plan ends first
+    r = ro(frame_target=2)
+    _frame_schedule(r, 1, 10)
+    assert r.frame_times == [1e-15, 2e-15]
+    shot = r.probe_shot()
+    assert shot["probe_frame_count"] == 2 and shot["probe_frame_indices"] == [1, 2]
+    assert all(v.shape == (0, 0, 0) for v in shot["frames"].values())
 
 
 def test_polarimetry_and_tr_summaries():
