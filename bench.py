@@ -39,8 +39,9 @@ sys.path.insert(0, os.path.join(ROOT, "magnetophotonic.jl_amd"))
 METRIC = "coupled Mcell-updates/s + % HBM roofline; surrogate vs plain at |Δm|≤1e-3"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64
-KERNELS = ("yee_H", "yee_E", "inject_plane", "ade_patch", "mo_gyration", "pabs", "film_epilogue", "multiphysics",
-           "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate", "dft_snapshot", "dft_batch")
+KERNELS = ("yee_H", "yee_HE", "yee_E", "pingpong_sync", "inject_plane", "ade_patch", "mo_gyration", "pabs",
+           "film_epilogue", "multiphysics", "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate",
+           "dft_snapshot", "dft_batch")
 DFT_BATCH = 32   # steps per batched DFT pass in the library
 
 
@@ -200,6 +201,8 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the form comparison (plain) / the relax accuracy study (surrogate)")
     ap.add_argument("--compare-steps", type=int, default=50, help="timed steps per form in the form comparison")
+    ap.add_argument("--pingpong", action="store_true",
+                    help="ping-pong steps: a second E/H buffer set and the fused interior H + E launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -226,7 +229,7 @@ def main():
         b.free()
     spec, meta = build_workload(args.workload, args.form, storage=args.storage)
     planes, bins = meta.get("planes", []), meta.get("omega_bins", [])
-    st = FDTDState(spec)
+    st = FDTDState(spec, pingpong=args.pingpong)
     if planes:
         st.attach_dft(DFTPlanes(st, planes, bins))
     L = _lib.lib()
@@ -243,7 +246,7 @@ def main():
     nx, ny, nz = spec.shape
     ncell = nx * ny * nz
     eps_idx = st.yee.n_eps > 0
-    kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH, st.fused_epilogue)
+    kb = roofline.step_bytes(spec, planes, len(bins), eps_idx, DFT_BATCH, st.fused_epilogue, args.pingpong)
     if "multiphysics" in kb:
         kb["surrogate"] = kb["multiphysics"]
     # the dominant kernel is the largest HBM consumer of the step (algorithmic bytes);
@@ -343,6 +346,7 @@ def main():
                    "llb_substeps": spec.subcycles, "cpml": list(spec.cpml.widths),
                    "inv_eps": f"uint8 index, {st.yee.n_eps} triples" if eps_idx else "FP32 volumes",
                    "film_list_work": "one fused pass" if st.fused_epilogue else "five launches",
+                   "yee_steps": "ping-pong pairs, fused interior H + E" if args.pingpong else "in place",
                    "dft_batch": DFT_BATCH},
         "roofline": {"bound": "hbm", "kernel": dom, "launches": dom_launches, "avg_us": round(dom_avg_us, 2),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -281,6 +281,16 @@ typedef struct mp_state {
      * runs it on a side stream beside the next step's Yee updates, joined before the
      * next gyration / absorbed-power kernels, which gives the same values. */
     int32_t film_serial;
+    /* Optional second buffer set of the six E and H volumes (ex, ey, ez, hx, hy, hz; same
+     * shape and type as the yee's).  When all six are set, the field type and arithmetic
+     * are FP32 on the vector kernels with an interior/CPML-shell split and the source is a
+     * soft or plane source on an E component (or none), mp_run_steps advances the steps in
+     * pairs of ping-pong steps: each reads E and H from one set and writes them to the
+     * other, so the interior H and E updates run as one fused launch (73 instead of 85
+     * bytes per interior cell).  Each pair ends with the fields back in the yee's volumes;
+     * an odd last step runs in place.  The results are those of the in-place sequence bit
+     * for bit.  The library keeps no pointer after the call.  NULL keeps the in-place steps. */
+    void *yee_alt[6];
 } mp_state;
 
 /* ---- runtime / backend hooks (src/core/Backend.jl:19-126) ---------------------------- */

@@ -10,6 +10,8 @@
 
 #include "common.h"
 #include "monitors.h"
+#include "sources.h"
+#include "yee.h"
 
 using namespace mp;
 
@@ -30,16 +32,39 @@ MP_API double mp_pulse_value(const mp_pulse *p, double t) {
     }
 }
 
-static int inject(mp_state *s) {
+// The source of the step, reading the field from `in` and writing it to `out` (the same
+// volumes except in a ping-pong step).
+static int inject(mp_state *s, const mp_yee *in, const mp_yee *out) {
     const mp_source &src = s->source;
     if (src.kind == MP_SRC_NONE) return MP_OK;
     const double v = mp_pulse_value(&src.pulse, s->t);
     switch (src.kind) {
-    case MP_SRC_SOFT: return mp_inject_soft_3d(s->yee, src.component, src.i, src.j, src.k, v);
-    case MP_SRC_PLANE: return mp_inject_plane_3d(s->yee, src.component, src.axis, src.i0, src.w0, src.i1, src.w1, v);
-    case MP_SRC_MODE_X: return mp_inject_mode_x_3d(s->yee, src.component, src.i, src.profile, v, src.hcorr);
+    case MP_SRC_SOFT: return inject_soft_pp(in, out, src.component, src.i, src.j, src.k, v);
+    case MP_SRC_PLANE: return inject_plane_pp(in, out, src.component, src.axis, src.i0, src.w0, src.i1, src.w1, v);
+    case MP_SRC_MODE_X: return mp_inject_mode_x_3d(in, src.component, src.i, src.profile, v, src.hcorr);
     default: return fail_arg("mp_run_steps: unknown source kind %d", src.kind);
     }
+}
+
+// Whether the steps may run as ping-pong pairs: a second buffer set, the fused Yee form for
+// this grid, and a source that writes only E and D (an H source, or the mode source's H
+// correction, lands between the H and E updates the fused launch joins).
+static bool pingpong_ok(const mp_state *s) {
+    for (int c = 0; c < 6; ++c)
+        if (!s->yee_alt[c]) return false;
+    const mp_source &src = s->source;
+    if (src.kind == MP_SRC_MODE_X) return false;
+    if ((src.kind == MP_SRC_SOFT || src.kind == MP_SRC_PLANE) && (src.component < MP_EX || src.component > MP_EZ))
+        return false;
+    return yee_fused_ok(s->yee, s->cpml);
+}
+
+// E volume `p` of the caller's yee as the same component of `y`.
+static void *same_component(const mp_yee *caller, const mp_yee *y, const void *p) {
+    if (p == caller->ex) return y->ex;
+    if (p == caller->ey) return y->ey;
+    if (p == caller->ez) return y->ez;
+    return const_cast<void *>(p);
 }
 
 // Side stream for the film integrator.  The 4TM + LLB update of step n is FP64-compute
@@ -90,53 +115,69 @@ static int launch_film_side(mp_state *s, double dt_mp) {
     return MP_OK;
 }
 
-static int film_list_work(mp_state *s);
+static int film_list_work(mp_state *s, const mp_yee *y);
 static int film_update_due(mp_state *s);
 
-static int one_step(mp_state *s) {
+// One step from the fields in `in` to the fields in `out`: in place (in == out, the
+// mp_update_H_3d / inject / mp_update_E_3d sequence) or ping-pong (the H shell, the
+// source, the fused interior H + E, the E shell).  Everything after the Yee updates reads
+// and writes `out`.
+static int one_step(mp_state *s, const mp_yee *in, const mp_yee *out) {
     int rc;
-    if ((rc = mp_update_H_3d(s->yee, s->cpml, s->dt, s->mu0))) return rc;
-    if ((rc = inject(s))) return rc;
-    if ((rc = mp_update_E_3d(s->yee, s->cpml, s->dt))) return rc;
+    if (in == out) {
+        if ((rc = mp_update_H_3d(in, s->cpml, s->dt, s->mu0))) return rc;
+        if ((rc = inject(s, in, in))) return rc;
+        if ((rc = mp_update_E_3d(in, s->cpml, s->dt))) return rc;
+    } else {
+        // E on the planes no update writes carries over first: the source and the film list
+        // work may have changed it there in the set this step reads
+        if ((rc = yee_sync_boundary(in, out, true))) return rc;
+        if ((rc = yee_fused_H_shell(in, out, s->cpml, s->dt, s->mu0))) return rc;
+        if ((rc = inject(s, in, out))) return rc;
+        if ((rc = yee_fused_interior(in, out, s->cpml, s->dt, s->mu0))) return rc;
+        if ((rc = yee_fused_E_shell(out, s->cpml, s->dt))) return rc;
+    }
+    const mp_yee *y = out;
     if (s->film_lists_coincide && s->ade[0] && s->film) {
         // the film list work in one pass; it contains the gyration, which reads the film
         // update of the previous step
         if ((rc = overlap_join())) return rc;
-        if ((rc = mp_film_epilogue(s->yee, s->ade[0], s->ade[1], s->ade[2], s->mo, s->mo_ea, s->mo_eb, s->mo_iea,
-                                   s->mo_ieb, s->film, s->has_ade_work, s->pabs_factor, s->Q_TM, s->Q_RE, s->dt)))
+        if ((rc = mp_film_epilogue(y, s->ade[0], s->ade[1], s->ade[2], s->mo, same_component(s->yee, y, s->mo_ea),
+                                   same_component(s->yee, y, s->mo_eb), s->mo_iea, s->mo_ieb, s->film,
+                                   s->has_ade_work, s->pabs_factor, s->Q_TM, s->Q_RE, s->dt)))
             return rc;
         s->n += 1;
         s->t += s->dt;
         if ((rc = film_update_due(s))) return rc;
     } else {
-        if ((rc = film_list_work(s))) return rc;
+        if ((rc = film_list_work(s, y))) return rc;
     }
     if (s->record_dft && s->nplanes > 0) {
-        if ((rc = dft_record_batched(s->yee, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t, s->dt)))
+        if ((rc = dft_record_batched(y, s->planes, s->nplanes, s->omega_bins, s->phase, s->nb, s->t, s->dt)))
             return rc;
     }
     return MP_OK;
 }
 
 // The list kernels after the E update, one launch each (the lists need not coincide).
-static int film_list_work(mp_state *s) {
+static int film_list_work(mp_state *s, const mp_yee *y) {
     int rc;
     if (s->ade[0]) {
-        void *E[3] = {s->yee->ex, s->yee->ey, s->yee->ez};
+        void *E[3] = {y->ex, y->ey, y->ez};
         for (int c = 0; c < 3; ++c)
-            if ((rc = mp_patch_E_dispersive(E[c], s->ade[c], s->yee->field_dtype, s->dt))) return rc;
+            if ((rc = mp_patch_E_dispersive(E[c], s->ade[c], y->field_dtype, s->dt))) return rc;
     }
     if ((rc = overlap_join())) return rc;   // the previous step's film update is complete
     if (s->mo && s->film) {
-        if ((rc = mp_patch_E_mo_gyration(s->mo_ea, s->mo_eb, s->mo, s->mo_iea, s->mo_ieb,
-                                         s->yee->field_dtype, s->film->m_TM_x, s->film->m_RE_x, s->Q_TM,
-                                         s->Q_RE, s->dt)))
+        if ((rc = mp_patch_E_mo_gyration(same_component(s->yee, y, s->mo_ea), same_component(s->yee, y, s->mo_eb),
+                                         s->mo, s->mo_iea, s->mo_ieb, y->field_dtype, s->film->m_TM_x,
+                                         s->film->m_RE_x, s->Q_TM, s->Q_RE, s->dt)))
             return rc;
     }
     s->n += 1;
     s->t += s->dt;
     if (s->film) {
-        if ((rc = mp_pabs_accumulate(s->film, s->yee, s->ade[0], s->ade[1], s->ade[2], s->has_ade_work,
+        if ((rc = mp_pabs_accumulate(s->film, y, s->ade[0], s->ade[1], s->ade[2], s->has_ade_work,
                                      s->pabs_factor, s->dt)))
             return rc;
         if ((rc = film_update_due(s))) return rc;
@@ -171,13 +212,24 @@ MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
     if (!s || !s->yee || !s->cpml) return fail_arg("mp_run_steps: null state");
     if (s->film && (!s->params || !s->lut)) return fail_arg("mp_run_steps: film state needs params and LUT");
     if (s->ade[0] && (!s->ade[1] || !s->ade[2])) return fail_arg("mp_run_steps: ADE needs all three lists");
-    for (int64_t k = 0; k < nsteps; ++k) {
-        int rc = one_step(s);
-        if (rc) {
-            if (s->record_dft && s->nplanes > 0) dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt);
-            overlap_join();
-            return rc;
-        }
+    // pairs of ping-pong steps (the yee's volumes -> the second set -> the yee's volumes),
+    // then any last step in place; the second set first takes the cells no update writes
+    const bool pp = nsteps >= 2 && pingpong_ok(s);
+    mp_yee alt = *s->yee;
+    int rc = MP_OK;
+    int64_t k = 0;
+    if (pp) {
+        alt.ex = s->yee_alt[0]; alt.ey = s->yee_alt[1]; alt.ez = s->yee_alt[2];
+        alt.hx = s->yee_alt[3]; alt.hy = s->yee_alt[4]; alt.hz = s->yee_alt[5];
+        rc = yee_sync_boundary(s->yee, &alt, false);
+        for (; !rc && k + 2 <= nsteps; k += 2)
+            if (!(rc = one_step(s, s->yee, &alt))) rc = one_step(s, &alt, s->yee);
+    }
+    for (; !rc && k < nsteps; ++k) rc = one_step(s, s->yee, s->yee);
+    if (rc) {
+        if (s->record_dft && s->nplanes > 0) dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt);
+        overlap_join();
+        return rc;
     }
     // DFT steps still held in the batch reach the accumulators before the call returns
     if (s->record_dft && s->nplanes > 0) {

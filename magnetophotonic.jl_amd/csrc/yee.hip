@@ -23,6 +23,7 @@
 #include <cstdlib>
 
 #include "common.h"
+#include "yee.h"
 
 namespace mp {
 
@@ -412,12 +413,15 @@ __device__ __forceinline__ bool lane_holds(int64_t o, int delta) {
     return !edge && (int64_t)(((uint64_t)nhi << 32) | nlo) == o + delta * VX;
 }
 
-template <typename P, int TYV>
+// PP: the updated H goes to Hxo/Hyo/Hzo (the other buffer set of a ping-pong step) instead of
+// back to Hx/Hy/Hz; the whole group is stored, so a cell at i = nx-1 carries its old value over.
+template <typename P, int TYV, bool PP>
 __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
     const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
-    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, ShellGeom sg) {
+    CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, ShellGeom sg,
+    float *__restrict__ Hxo, float *__restrict__ Hyo, float *__restrict__ Hzo) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
     const LaneGroup lg = lane_group(sg, nx, ny, nbx, TYV);
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
@@ -496,9 +500,9 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
         el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + pHzx) - (dEx_dy * yk_ + pHzy));
     }
     if (act) {
-        st4nt(Hx + o, hx);
-        st4nt(Hy + o, hy);
-        st4nt(Hz + o, hz);
+        st4nt((PP ? Hxo : Hx) + o, hx);
+        st4nt((PP ? Hyo : Hy) + o, hy);
+        st4nt((PP ? Hzo : Hz) + o, hz);
         psix_store(px, ps, xslab);
         if (ry) {
             psi4_store(pxy, oy0, qxy);
@@ -820,6 +824,231 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
     }
 }
 
+// The six E and H volumes of one buffer set (ex, ey, ez, hx, hy, hz).
+struct PlanePtrs {
+    float *p[6];
+};
+
+// ---- fused interior H + E (ping-pong) -----------------------------------------------------
+// One pass over the interior box for a whole step: H^{n+1/2} and then E^{n+1}, reading E^n and
+// H^{n-1/2} from one buffer set and writing H^{n+1/2}, E^{n+1} to the other (D in place), so
+// no value a workgroup reads is written during the launch.  A workgroup owns T = FL - ngi
+// consecutive float4 groups of a box plane (rows flattened, row after row) and marches them
+// through a chunk of planes.  Its lanes 0..ngi-1 take the ngi groups just before (the row
+// below each of its first ngi groups): they recompute that row's H, which the E update needs
+// as its y neighbour, from the same unmodified inputs, so the values equal the owner's bit for
+// bit.  Each plane: E^n of the plane is staged in LDS, every lane forms its group's H from its
+// own and its neighbours' E (x and y neighbours from LDS, z neighbour from registers), the new
+// H is staged in LDS, and the owning lanes form D and E from it (x and y neighbours from LDS,
+// the z neighbour kept from the previous plane).  Neighbours outside the box are CPML-shell
+// cells: their new H was written by the shell launch that runs first, and their E^n is read
+// from the input set.  A chunk that starts inside the box first recomputes the new H of the
+// plane below it.  Per cell this reads E, H, D and the material byte once and writes H, D and
+// E once: 73 bytes instead of the 85 of the two interior launches.  The per-cell arithmetic is
+// that of yee_H_int_kernel and yee_E_int_kernel, operation for operation.
+constexpr int FL = 512;   // lanes per workgroup (8 waves: two fit each SIMD beside the film update)
+
+// Loads and stores at a 32-bit byte offset from a uniform base: every volume of the fused
+// kernel is then addressed by one offset register and its base in scalar registers.
+template <typename T>
+__device__ __forceinline__ const T &at(const float *base, uint32_t boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+__device__ __forceinline__ float4 ldb4(const float *base, uint32_t boff) { return at<float4>(base, boff); }
+__device__ __forceinline__ float4 ldb4nt(const float *base, uint32_t boff) {
+    return ld4nt(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff));
+}
+__device__ __forceinline__ void stb4nt(float *base, uint32_t boff, float4 v) {
+    st4nt(reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff), v);
+}
+
+template <bool MI>
+__global__ __launch_bounds__(FL) void yee_HE_int_kernel(
+    const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
+    const float *__restrict__ Hx, const float *__restrict__ Hy, const float *__restrict__ Hz,
+    float *__restrict__ Exo, float *__restrict__ Eyo, float *__restrict__ Ezo,
+    float *__restrict__ Hxo, float *__restrict__ Hyo, float *__restrict__ Hzo,
+    float *__restrict__ Dx, float *__restrict__ Dy, float *__restrict__ Dz,
+    const float *__restrict__ iex, const float *__restrict__ iey, const float *__restrict__ iez,
+    const uint8_t *__restrict__ mat, const float4 *__restrict__ mtab, int nmat,
+    const float *__restrict__ idcx, const float *__restrict__ idcy, const float *__restrict__ idcz,
+    const float *__restrict__ iddx, const float *__restrict__ iddy, const float *__restrict__ iddz,
+    const float *__restrict__ xk, const float *__restrict__ yk, const float *__restrict__ zk, uint32_t nx,
+    uint32_t ny, uint32_t gi0, uint32_t ngi, uint32_t j0, uint32_t nplane, uint32_t k0, uint32_t kend,
+    uint32_t tiles, uint32_t kchunk, float dt_mu0, float dt) {
+    __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
+    __shared__ float4 sE[3][FL];
+    __shared__ float4 sH[3][FL];
+    __shared__ float4 tab[MI ? 256 : 1];
+    const uint32_t l = threadIdx.x;
+    if (MI)   // read after the first barrier of the plane loop
+        for (int t = (int)l; t < nmat; t += FL) tab[t] = mtab[t];
+    const uint32_t tile = blockIdx.x % tiles, chunk = blockIdx.x / tiles;
+    const int64_t f = (int64_t)tile * (FL - ngi) - (int64_t)ngi + (int64_t)l;   // group of the box plane
+    const bool valid = f >= 0 && f < (int64_t)nplane;
+    const bool own = valid && l >= ngi;
+    const uint32_t fc = valid ? (uint32_t)f : 0u;
+    const uint32_t jj = fc / ngi, gg = fc - jj * ngi;
+    const uint32_t i0 = (gi0 + gg) * VX, j = j0 + jj;
+    // byte offsets: x step 4, row step sy, plane step sz
+    const uint32_t sy = 4u * nx, sz = sy * ny;
+    const uint32_t brow = 4u * i0 + sy * j;
+    const uint32_t kc0 = k0 + chunk * kchunk, kc1 = min(kc0 + kchunk, kend);
+    const float idcy_ = idcy[j], iddy_ = iddy[j], yk_ = yk[j];
+    // E^n of the current plane and the new H of the plane below (own groups)
+    uint32_t k = kc0 == k0 ? kc0 : kc0 - 1;
+    float4 ecx = make_float4(0.f, 0.f, 0.f, 0.f), ecy = ecx, ecz = ecx, hpx = ecx, hpy = ecx;
+    if (valid) {
+        const uint32_t o = brow + sz * k;
+        ecx = ldb4(Ex, o); ecy = ldb4(Ey, o); ecz = ldb4(Ez, o);
+    }
+    if (kc0 == k0 && own) {   // the plane below the box is CPML shell, updated by the shell launch
+        hpx = ldb4(Hxo, brow + sz * (k0 - 1));
+        hpy = ldb4(Hyo, brow + sz * (k0 - 1));
+    }
+    for (; k < kc1; ++k) {
+        const bool upd = k >= kc0;   // false on the recomputed plane below the chunk
+        const uint32_t o = brow + sz * k;
+        float4 enx = ecx, eny = ecy, enz = ecz, hx = ecx, hy = ecy, hz = ecz;
+        if (valid) {
+            enx = ldb4(Ex, o + sz); eny = ldb4(Ey, o + sz); enz = ldb4(Ez, o + sz);
+            hx = ldb4nt(Hx, o); hy = ldb4nt(Hy, o); hz = ldb4nt(Hz, o);
+        }
+        sE[0][l] = ecx; sE[1][l] = ecy; sE[2][l] = ecz;
+        __syncthreads();
+        // D and the material of the plane, for the E half: issued here, after the barrier, so
+        // they are in flight during the H arithmetic without holding registers across it
+        float4 dx = ecx, dy = ecy, dz = ecz, iex4 = ecx, iey4 = ecy, iez4 = ecz;
+        uint32_t m4 = 0u;
+        if (upd && own) {
+            dx = ldb4nt(Dx, o); dy = ldb4nt(Dy, o); dz = ldb4nt(Dz, o);
+            if (MI) {
+                m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + (o >> 2)));
+            } else {
+                iex4 = ldb4nt(iex, o); iey4 = ldb4nt(iey, o); iez4 = ldb4nt(iez, o);
+            }
+        }
+        if (valid) {
+            // y neighbour (next row): the lane ngi on, or memory past the workgroup / the box
+            float4 ez_y, ex_y;
+            if (l + ngi < (uint32_t)FL && f + ngi < (int64_t)nplane) {
+                ez_y = sE[2][l + ngi];
+                ex_y = sE[0][l + ngi];
+            } else {
+                ez_y = ldb4(Ez, o + sy);
+                ex_y = ldb4(Ex, o + sy);
+            }
+            // x neighbour of element 3: the next lane's group in this row, or memory
+            float ez_nx, ey_nx;
+            if (gg + 1 < ngi && l + 1 < (uint32_t)FL) {
+                ez_nx = sE[2][l + 1].x;
+                ey_nx = sE[1][l + 1].x;
+            } else {
+                ez_nx = at<float>(Ez, o + 4u * VX);
+                ey_nx = at<float>(Ey, o + 4u * VX);
+            }
+            const float idz_ = idcz[k], zk_ = zk[k];
+            const float4 idc4 = ldb4(idcx, 4u * i0), xk4 = ldb4(xk, 4u * i0);
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const float ezc = elc(ecz, e), eyc = elc(ecy, e), exc = elc(ecx, e);
+                const float ez_xp = e < VX - 1 ? elc(ecz, e + 1) : ez_nx;
+                const float ey_xp = e < VX - 1 ? elc(ecy, e + 1) : ey_nx;
+                const float idx_ = elc(idc4, e), xk_ = elc(xk4, e);
+                const float dEz_dy = (elc(ez_y, e) - ezc) * idcy_;
+                const float dEy_dz = (elc(eny, e) - eyc) * idz_;
+                const float dEx_dz = (elc(enx, e) - exc) * idz_;
+                const float dEz_dx = (ez_xp - ezc) * idx_;
+                const float dEy_dx = (ey_xp - eyc) * idx_;
+                const float dEx_dy = (elc(ex_y, e) - exc) * idcy_;
+                el(hx, e) = el(hx, e) - dt_mu0 * ((dEz_dy * yk_ + 0.0f) - (dEy_dz * zk_ + 0.0f));
+                el(hy, e) = el(hy, e) - dt_mu0 * ((dEx_dz * zk_ + 0.0f) - (dEz_dx * xk_ + 0.0f));
+                el(hz, e) = el(hz, e) - dt_mu0 * ((dEy_dx * xk_ + 0.0f) - (dEx_dy * yk_ + 0.0f));
+            }
+        }
+        sH[0][l] = hx; sH[1][l] = hy; sH[2][l] = hz;
+        __syncthreads();
+        if (upd && own) {
+            // y neighbour (previous row): the lane ngi back (the recomputed row for the first
+            // ngi groups), or the shell row below the box
+            float4 hz_y, hx_y;
+            if (jj > 0) {
+                hz_y = sH[2][l - ngi];
+                hx_y = sH[0][l - ngi];
+            } else {
+                hz_y = ldb4(Hzo, o - sy);
+                hx_y = ldb4(Hxo, o - sy);
+            }
+            // x neighbour of element 0: the previous lane's group in this row, or the shell
+            float hz_px, hy_px;
+            if (gg > 0) {
+                hz_px = sH[2][l - 1].w;
+                hy_px = sH[1][l - 1].w;
+            } else {
+                hz_px = at<float>(Hzo, o - 4u);
+                hy_px = at<float>(Hyo, o - 4u);
+            }
+            const float idz_ = iddz[k], zk_ = zk[k];
+            const float4 idd4 = ldb4(iddx, 4u * i0), xk4 = ldb4(xk, 4u * i0);
+#pragma unroll
+            for (int e = 0; e < VX; ++e) {
+                const float hxc = elc(hx, e), hyc = elc(hy, e), hzc = elc(hz, e);
+                const float hz_xm = e > 0 ? elc(hz, e - 1) : hz_px;
+                const float hy_xm = e > 0 ? elc(hy, e - 1) : hy_px;
+                const float idx_ = elc(idd4, e), xk_ = elc(xk4, e);
+                const float dHz_dy = (hzc - elc(hz_y, e)) * iddy_;
+                const float dHy_dz = (hyc - elc(hpy, e)) * idz_;
+                const float dHx_dz = (hxc - elc(hpx, e)) * idz_;
+                const float dHz_dx = (hzc - hz_xm) * idx_;
+                const float dHy_dx = (hyc - hy_xm) * idx_;
+                const float dHx_dy = (hxc - elc(hx_y, e)) * iddy_;
+                const float cx_term = (dHz_dy * yk_ + 0.0f) - (dHy_dz * zk_ + 0.0f);
+                const float cy_term = (dHx_dz * zk_ + 0.0f) - (dHz_dx * xk_ + 0.0f);
+                const float cz_term = (dHy_dx * xk_ + 0.0f) - (dHx_dy * yk_ + 0.0f);
+                el(dx, e) = el(dx, e) + dt * cx_term;
+                el(dy, e) = el(dy, e) + dt * cy_term;
+                el(dz, e) = el(dz, e) + dt * cz_term;
+            }
+            if (MI) {
+                const float4 t0 = tab[m4 & 0xffu], t1 = tab[(m4 >> 8) & 0xffu];
+                const float4 t2 = tab[(m4 >> 16) & 0xffu], t3 = tab[m4 >> 24];
+                iex4 = make_float4(t0.x, t1.x, t2.x, t3.x);
+                iey4 = make_float4(t0.y, t1.y, t2.y, t3.y);
+                iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
+            }
+            stb4nt(Hxo, o, hx); stb4nt(Hyo, o, hy); stb4nt(Hzo, o, hz);
+            stb4nt(Dx, o, dx); stb4nt(Dy, o, dy); stb4nt(Dz, o, dz);
+            stb4nt(Exo, o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
+            stb4nt(Eyo, o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
+            stb4nt(Ezo, o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
+        }
+        hpx = hx; hpy = hy;
+        ecx = enx; ecy = eny; ecz = enz;
+    }
+}
+
+// The cells no update writes — E on the planes i = 0, j = 0, k = 0 and H on the planes
+// i = nx-1, j = ny-1, k = nz-1 — copied from one buffer set to the other.  blockIdx.y: 0-2
+// the E planes normal to x, y, z; 3-5 the H planes (a grid of 3 copies E only).
+__global__ __launch_bounds__(256) void sync_boundary_kernel(PlanePtrs a, PlanePtrs b, int64_t nx, int64_t ny,
+                                                            int64_t nz) {
+    const int q = blockIdx.y, axis = q % 3;
+    const bool h = q >= 3;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n1 = axis == 0 ? ny : nx, n2 = axis == 2 ? ny : nz;
+    if (c >= n1 * n2) return;
+    const int64_t u = c % n1, v = c / n1;
+    const int64_t idx = h ? (axis == 0 ? nx - 1 : (axis == 1 ? ny - 1 : nz - 1)) : 0;
+    int64_t o;
+    if (axis == 0) o = idx + nx * (u + ny * v);
+    else if (axis == 1) o = u + nx * (idx + ny * v);
+    else o = u + nx * (v + ny * idx);
+    for (int comp = 0; comp < 3; ++comp) {
+        const int s = (h ? 3 : 0) + comp;
+        b.p[s][o] = a.p[s][o];
+    }
+}
+
 // z-chunk length: enough workgroups to fill 256 CUs several times over.
 static int64_t pick_kchunk(int64_t nx, int64_t ny, int64_t nz) {
     const int64_t tiles = (int64_t)cdiv(nx, TX) * (int64_t)cdiv(ny, TY);
@@ -922,22 +1151,36 @@ static ShellGeom shell_geom(const mp_yee *g, const Box &b) {
     return sg;
 }
 
+// pp (ping-pong step): only the CPML-shell launch, from g's E and H into pp's H; the interior
+// is then updated by the fused kernel.
 template <typename P, int TY>
-static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double mu0) {
+static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double mu0, const mp_yee *pp = nullptr) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:60-101)
     const float dt_mu0 = (float)dt / (float)mu0;
     const Box b = interior_box(g, c);
     const ShellGeom sg = split_ok(g, b) ? shell_geom(g, b) : ShellGeom{};
     dim3 block(64, TY, 1);
+    if (pp) {
+        if (!sg.n) return fail_arg("fused Yee step: no interior/shell split for this grid");
+        KernelTimer t("yee_H");
+        yee_H_vec_kernel<P, TY, true><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
+            (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+            (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sg, (float *)pp->hx,
+            (float *)pp->hy, (float *)pp->hz);
+        MP_LAUNCH_CHECK("yee_H_vec_kernel");
+        return MP_OK;
+    }
     if (!sg.n) {
         // one launch: one z-plane per workgroup range (nbx workgroups per plane)
         const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
         KernelTimer t("yee_H");
-        yee_H_vec_kernel<P, TY><<<(unsigned)(nbx * nbz), block, 0, stream()>>>(
+        yee_H_vec_kernel<P, TY, false><<<(unsigned)(nbx * nbz), block, 0, stream()>>>(
             (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
             (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, ShellGeom{});
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, nbx, ShellGeom{},
+            nullptr, nullptr, nullptr);
         MP_LAUNCH_CHECK("yee_H_vec_kernel");
         return MP_OK;
     }
@@ -949,16 +1192,18 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         (const float *)c->inv_kappa[2], nx, ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0,
         (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0, (uint32_t)b.n(), dt_mu0);
     MP_LAUNCH_CHECK("yee_H_int_kernel");
-    yee_H_vec_kernel<P, TY><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
+    yee_H_vec_kernel<P, TY, false><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
         (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sg);
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sg, nullptr, nullptr,
+        nullptr);
     MP_LAUNCH_CHECK("yee_H_vec_kernel");
     return MP_OK;
 }
 
+// shell_only (ping-pong step): only the CPML-shell launch; the fused kernel did the interior.
 template <typename P, int TY>
-static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
+static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt, bool shell_only = false) {
     const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
     if (ny < 2 || nz < 2) return MP_OK;   // empty update range (Kernels.jl:103-152)
     const bool mi = g->eps_index && g->eps_table && g->n_eps > 0 && g->n_eps <= 256;
@@ -971,6 +1216,7 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
         (const float *)g->inv_eps_y, (const float *)g->inv_eps_z, g->eps_index,                            \
         (const float4 *)g->eps_table, g->n_eps, (const float *)g->inv_d_dual_x,                            \
         (const float *)g->inv_d_dual_y, (const float *)g->inv_d_dual_z
+    if (shell_only && !sg.n) return fail_arg("fused Yee step: no interior/shell split for this grid");
     if (!sg.n) {
         // one launch: one z-plane per workgroup range (nbx workgroups per plane)
         const int64_t nbx = cdiv((nx / VX) * ny, 64 * TY), nbz = nz;
@@ -990,9 +1236,11 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt) {
     (const float *)c->inv_kappa[0], (const float *)c->inv_kappa[1], (const float *)c->inv_kappa[2], nx, ny, \
         (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0, (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0,         \
         (uint32_t)b.n(), (float)dt
-    if (mi) yee_E_int_kernel<true><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
-    else yee_E_int_kernel<false><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
-    MP_LAUNCH_CHECK("yee_E_int_kernel");
+    if (!shell_only) {
+        if (mi) yee_E_int_kernel<true><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
+        else yee_E_int_kernel<false><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
+        MP_LAUNCH_CHECK("yee_E_int_kernel");
+    }
     const unsigned grid = cdiv(sg.n, 64 * TY);
     if (mi)
         yee_E_vec_kernel<P, TY, true, true><<<grid, block, 0, stream()>>>(
@@ -1064,6 +1312,81 @@ static int launch_E(const mp_yee *g, const mp_cpml *c, double dt) {
         (const S *)g->inv_eps_z, (const C *)g->inv_d_dual_x, (const C *)g->inv_d_dual_y,
         (const C *)g->inv_d_dual_z, make_cpml_args(c, false), nx, ny, nz, kc, (C)dt);
     MP_LAUNCH_CHECK("yee_E_kernel");
+    return MP_OK;
+}
+
+// ---- ping-pong steps for the step driver -------------------------------------------------
+bool yee_fused_ok(const mp_yee *g, const mp_cpml *c) {
+    if (!g || !c || check_yee(g, c, "yee_fused_ok") != MP_OK) return false;
+    if (g->field_dtype != MP_F32 || g->compute_dtype != MP_F32 || g->kernel_form != MP_YEE_AUTO) return false;
+    if (!(g->nx > 1 && g->ny > 1 && g->nz > 1)) return false;
+    if (c->psi_dtype != MP_F32 && c->psi_dtype != MP_BF16 && c->psi_dtype != MP_F16) return false;
+    if (!vec_ok(g, c, true) || !vec_ok(g, c, false)) return false;
+    const Box b = interior_box(g, c);
+    if (!split_ok(g, b) || !shell_geom(g, b).n) return false;
+    // the recomputed row of a workgroup (ngi lanes) must leave room for as many owned lanes;
+    // the fused kernel addresses the volumes by 32-bit byte offsets
+    return b.ngi >= 1 && 2 * b.ngi <= FL && 4 * g->nx * g->ny * g->nz < ((int64_t)1 << 32);
+}
+
+int yee_fused_H_shell(const mp_yee *in, const mp_yee *out, mp_cpml *c, double dt, double mu0) {
+    switch (c->psi_dtype) {
+    case MP_F32: return launch_H_vec_t<float, TYV>(in, c, dt, mu0, out);
+    case MP_BF16: return launch_H_vec_t<bf16, TYV>(in, c, dt, mu0, out);
+    case MP_F16: return launch_H_vec_t<f16s, TYV>(in, c, dt, mu0, out);
+    default: return fail_arg("fused Yee step: unsupported psi dtype %d", c->psi_dtype);
+    }
+}
+
+int yee_fused_E_shell(const mp_yee *out, mp_cpml *c, double dt) {
+    switch (c->psi_dtype) {
+    case MP_F32: return launch_E_vec_t<float, TYV>(out, c, dt, true);
+    case MP_BF16: return launch_E_vec_t<bf16, TYV>(out, c, dt, true);
+    case MP_F16: return launch_E_vec_t<f16s, TYV>(out, c, dt, true);
+    default: return fail_arg("fused Yee step: unsupported psi dtype %d", c->psi_dtype);
+    }
+}
+
+int yee_fused_interior(const mp_yee *in, const mp_yee *out, const mp_cpml *c, double dt, double mu0) {
+    const Box b = interior_box(in, c);
+    const int64_t nplane = b.ngi * b.nj, T = FL - b.ngi;
+    const int64_t tiles = cdiv(nplane, T);
+    // about 1024 workgroups (two to four per CU), planes per chunk at least 8
+    const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>(cdiv(1024, tiles), b.nk / 8));
+    const int64_t kchunk = cdiv(b.nk, chunks);
+    const int64_t nwg = tiles * cdiv(b.nk, kchunk);
+    if (nwg >= ((int64_t)1 << 31)) return fail_arg("fused Yee step: grid too large");
+    const bool mi = in->eps_index && in->eps_table && in->n_eps > 0 && in->n_eps <= 256;
+    const float dt_mu0 = (float)dt / (float)mu0;
+    KernelTimer t("yee_HE");
+#define HE_ARGS                                                                                                \
+    (const float *)in->ex, (const float *)in->ey, (const float *)in->ez, (const float *)in->hx,                \
+        (const float *)in->hy, (const float *)in->hz, (float *)out->ex, (float *)out->ey, (float *)out->ez,    \
+        (float *)out->hx, (float *)out->hy, (float *)out->hz, (float *)in->dx, (float *)in->dy,                 \
+        (float *)in->dz, (const float *)in->inv_eps_x, (const float *)in->inv_eps_y,                            \
+        (const float *)in->inv_eps_z, in->eps_index, (const float4 *)in->eps_table, in->n_eps,                 \
+        (const float *)in->inv_d_cell_x, (const float *)in->inv_d_cell_y, (const float *)in->inv_d_cell_z,     \
+        (const float *)in->inv_d_dual_x, (const float *)in->inv_d_dual_y, (const float *)in->inv_d_dual_z,     \
+        (const float *)c->inv_kappa[0], (const float *)c->inv_kappa[1], (const float *)c->inv_kappa[2],        \
+        (uint32_t)in->nx, (uint32_t)in->ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0, (uint32_t)nplane, \
+        (uint32_t)b.k0, (uint32_t)(b.k0 + b.nk), (uint32_t)tiles, (uint32_t)kchunk, dt_mu0, (float)dt
+    if (mi) yee_HE_int_kernel<true><<<(unsigned)nwg, FL, 0, stream()>>>(HE_ARGS);
+    else yee_HE_int_kernel<false><<<(unsigned)nwg, FL, 0, stream()>>>(HE_ARGS);
+#undef HE_ARGS
+    MP_LAUNCH_CHECK("yee_HE_int_kernel");
+    return MP_OK;
+}
+
+int yee_sync_boundary(const mp_yee *from, const mp_yee *to, bool e_only) {
+    const int64_t nx = from->nx, ny = from->ny, nz = from->nz;
+    const PlanePtrs a{{(float *)from->ex, (float *)from->ey, (float *)from->ez, (float *)from->hx,
+                       (float *)from->hy, (float *)from->hz}};
+    const PlanePtrs bb{{(float *)to->ex, (float *)to->ey, (float *)to->ez, (float *)to->hx, (float *)to->hy,
+                        (float *)to->hz}};
+    const int64_t n = std::max(ny * nz, std::max(nx * nz, nx * ny));
+    KernelTimer t("pingpong_sync");
+    sync_boundary_kernel<<<dim3(cdiv(n, 256), e_only ? 3 : 6), 256, 0, stream()>>>(a, bb, nx, ny, nz);
+    MP_LAUNCH_CHECK("sync_boundary_kernel");
     return MP_OK;
 }
 

@@ -233,6 +233,7 @@ struct MpState
     has_ade_work::Int32; record_dft::Int32
     surrogate::Ptr{MpMlp}; surrogate_tol::Cdouble; surrogate_fallbacks::Ptr{Int64}
     film_lists_coincide::Int32; film_serial::Int32
+    yee_alt::NTuple{6,Ptr{Cvoid}}
 end
 
 _p(x::HIPArray) = Ptr{Cvoid}(x.ptr)
@@ -670,7 +671,7 @@ function _run_steps!(state::FDTDState, steps::Integer)
                          Int32(state.multiphysics_enabled && film_on), Int32(hw), Int32(0),
                          mlp === nothing ? Ptr{MpMlp}(C_NULL) : ptr(mlp), sur === nothing ? 0.0 : sur[2],
                          sur === nothing ? Ptr{Int64}(C_NULL) : sur[3],
-                         Int32(film_on && _lists_coincide(state)), Int32(0)))
+                         Int32(film_on && _lists_coincide(state)), Int32(0), ntuple(_ -> C_NULL, 6)))
         _check(ccall((:mp_run_steps, libmphip), Cint, (Ref{MpState}, Int64), st, Int64(steps)), "mp_run_steps")
         state.n = Int(st[].n)
         state.t = st[].t

@@ -92,7 +92,7 @@ class State(C.Structure):
                 ("multiphysics_every", i32), ("subcycles", i32), ("brillouin_iters", i32),
                 ("multiphysics_enabled", i32), ("has_ade_work", i32), ("record_dft", i32),
                 ("surrogate", C.POINTER(Mlp)), ("surrogate_tol", f64), ("surrogate_fallbacks", vp),
-                ("film_lists_coincide", i32), ("film_serial", i32)]
+                ("film_lists_coincide", i32), ("film_serial", i32), ("yee_alt", vp * 6)]
 
 
 MP_PULSE_GAUSSIAN, MP_PULSE_CARRIER_GAUSSIAN, MP_PULSE_CONTINUOUS = 1, 2, 3

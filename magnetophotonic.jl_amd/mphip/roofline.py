@@ -32,7 +32,18 @@ def _psi_cells(spec) -> int:
     return tot
 
 
-def kernel_bytes(spec, eps_index: bool = False, fused_epilogue: bool = False) -> dict:
+def interior_cells(spec) -> int:
+    """Cells of the interior box the split Yee launches give their interior kernels: whole
+    float4 groups off every CPML slab and boundary cell, x edges on 128-byte lines."""
+    nx, ny, nz = spec.shape
+    mx, my, mz = (max(int(w), 1) for w in spec.cpml.widths)
+    vx = 4
+    gi0 = ((mx + vx - 1) // vx + 7) // 8 * 8
+    ngi = max((nx - mx) // vx // 8 * 8 - gi0, 0)
+    return ngi * vx * max(ny - 2 * my, 0) * max(nz - 2 * mz, 0)
+
+
+def kernel_bytes(spec, eps_index: bool = False, fused_epilogue: bool = False, pingpong: bool = False) -> dict:
     """Algorithmic bytes per launch of each kernel of one step.  eps_index: the E update
     reads inv_eps as a 1-byte material index per cell instead of three volumes.
     fused_epilogue: the ADE, gyration and absorbed-power list work runs as one pass over
@@ -44,8 +55,18 @@ def kernel_bytes(spec, eps_index: bool = False, fused_epilogue: bool = False) ->
     n_upd = (nx - 1) * (ny - 1) * (nz - 1)
     out = {}
     # H: read E(3), H(3); write H(3).  E: read H(3), D(3), inv_eps(3); write D(3), E(3).
+    e_cell = (12 * fs + 1) if eps_index else 15 * fs
     out["yee_H"] = 9 * fs * n_upd + 2 * ps * _psi_cells(spec)
-    out["yee_E"] = ((12 * fs + 1) if eps_index else 15 * fs) * n_upd + 2 * ps * _psi_cells(spec)
+    out["yee_E"] = e_cell * n_upd + 2 * ps * _psi_cells(spec)
+    if pingpong:
+        # the interior as one fused launch (read E, H, D and the material, write H, D, E);
+        # the H and E launches take the CPML shell only; E on the three planes no update
+        # writes is carried to the other buffer set each step
+        n_int = interior_cells(spec)
+        out["yee_HE"] = (6 * fs + e_cell) * n_int      # the E update's H read is the H just formed
+        out["yee_H"] -= 9 * fs * n_int
+        out["yee_E"] -= e_cell * n_int
+        out["pingpong_sync"] = 3 * 2 * fs * (ny * nz + nx * nz + nx * ny)
     if spec.ade is not None:
         np_ = len(spec.diag_poles) // 3
         ss = _SZ[spec.ade_state_dtype]
@@ -89,11 +110,11 @@ def plane_cells(spec, planes) -> int:
 
 
 def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch: int = 1,
-               fused_epilogue: bool = False) -> dict:
+               fused_epilogue: bool = False, pingpong: bool = False) -> dict:
     """Algorithmic bytes per step.  With dft_batch > 1 the DFT record is a per-step
     snapshot (4 fields read, 4 written) plus, once per batch, the stored steps read and
     the accumulators read and written once."""
-    kb = kernel_bytes(spec, eps_index, fused_epilogue)
+    kb = kernel_bytes(spec, eps_index, fused_epilogue, pingpong)
     if planes and nb:
         if dft_batch > 1:
             fs = spec.field_dtype.itemsize

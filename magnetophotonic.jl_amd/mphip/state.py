@@ -87,11 +87,15 @@ class FDTDState:
     at most 256 distinct triples (False: it reads the three inv_eps volumes, as the Julia
     extension's launcher does).  yee_form: "auto", "whole" or "scalar" kernel family of
     the FP32 Yee updates.  film_serial: run the film update in step order on the launch
-    stream instead of beside the next step's Yee updates.  All of these give the same
-    values."""
+    stream instead of beside the next step's Yee updates.  pingpong: allocate a second set
+    of the six E and H volumes, with which `run` advances FP32 states in pairs of ping-pong
+    steps whose interior H and E updates are one fused launch (where the grid and source
+    allow it; the library falls back to in-place steps otherwise).  All of these give the
+    same values.  Measured on the 512 x 512 x 256 pump step the ping-pong form is slower
+    (1.78 against 1.52 ms), so it is off by default."""
 
     def __init__(self, spec: StateSpec, device: int = 0, use_eps_index: bool = True, yee_form: str = "auto",
-                 film_serial: bool = False):
+                 film_serial: bool = False, pingpong: bool = False):
         _lib.init(device)
         self.spec = spec
         self._st = abi.State()
@@ -137,6 +141,9 @@ class FDTDState:
             self._eps_index, self._eps_table = dev_arr(ei[0]), dev_arr(ei[1])
             y.eps_index, y.eps_table, y.n_eps = self._eps_index.ptr, self._eps_table.ptr, ei[1].shape[0]
         self.yee = y
+        self.yee_alt = None
+        if pingpong and FT == np.float32 and CT == np.float32 and yee_form == "auto":
+            self.yee_alt = [dev_zeros(ncell * FT.itemsize) for _ in range(6)]
 
         # CPML
         cs = spec.cpml
@@ -352,6 +359,8 @@ class FDTDState:
         st.has_ade_work = 1 if spec.has_ade_work() else 0
         st.film_lists_coincide = 1 if self._lists_coincide() else 0
         st.film_serial = 1 if self._film_serial else 0
+        for c in range(6):
+            st.yee_alt[c] = self.yee_alt[c].ptr if self.yee_alt is not None else None
 
     def _lists_coincide(self) -> bool:
         """Every ADE list and the MO list is the film list (entry n is film cell n) and the

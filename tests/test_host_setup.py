@@ -78,6 +78,23 @@ def test_roofline_bulk_bytes():
     assert kb["yee_E"] - 60 * n_upd == kb["yee_H"] - 36 * n_upd   # same psi slab traffic
 
 
+def test_roofline_pingpong_bytes():
+    """The fused interior launch reads E, H, D and the material byte and writes H, D, E once
+    per cell: 73 bytes with the index (84 with the volumes), 12 fewer than the two interior
+    launches, which the H and E totals lose on the interior box.  The box of the 96 x 68 x 64
+    pump with CPML 6: x groups 8..15 (32 cells), rows 6..61, planes 6..57."""
+    from mphip import configs, roofline
+    spec, _ = configs.coupled_pump(nx=96, ny=68, nz=64, steps=10, npml=6, src_z=40e-9)
+    n = roofline.interior_cells(spec)
+    assert n == 32 * 56 * 52
+    for idx, per in ((True, 73), (False, 84)):
+        a = roofline.kernel_bytes(spec, idx)
+        b = roofline.kernel_bytes(spec, idx, pingpong=True)
+        assert b["yee_HE"] == per * n
+        assert b["yee_H"] + b["yee_HE"] + b["yee_E"] == a["yee_H"] + a["yee_E"] - 12 * n
+        assert b["pingpong_sync"] == 3 * 2 * 4 * (68 * 64 + 96 * 64 + 96 * 68)
+
+
 def test_measured_figures_follow_the_kernel_sources(monkeypatch):
     """Counter figures are reported only for the kernel sources they were measured on: an
     entry whose digest is not the current sources' gives (None, None)."""
