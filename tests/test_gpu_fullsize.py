@@ -181,3 +181,53 @@ def test_surrogate_step_full_film():
             assert np.median(np.abs(got[k][sample].astype(np.float64) - fa[k])) < 1e-6, k
         gpu.free()
     sur.free()
+
+
+@pytest.mark.timeout(900)
+def test_coupled_pump_workload_from_rest():
+    """The 512 x 512 x 256 pump workload itself, not seeded fields: from rest with the pump
+    pulse already on (t0 is half the 20 000-step run, inside the pulse), 40 steps of the
+    step driver with its batched DFT record on the 6 planes x 4 bins, against the oracle
+    stepping and recording every step.  Fields within 1e-5 relative L2 (the gyration feeds
+    the FP64 film state back into E), film to the film bar, DFT accumulators to 1e-10
+    relative L2 (cos/sin of the two math libraries)."""
+    import driver
+    from mphip import configs
+    from mphip.monitors import DFTPlanes
+    from mphip.state import FDTDState
+    spec, meta = configs.coupled_pump(nx=512, ny=512, nz=256, steps=20000)
+    print("case coupled pump 512x512x256 from rest, 40 steps, DFT on", flush=True)
+    gpu, ref = FDTDState(spec), driver.OracleState(spec)
+    mon = DFTPlanes(gpu, meta["planes"], meta["omega_bins"])
+    gpu.attach_dft(mon)
+    ref_mon = driver.OracleDFT(ref, meta["planes"], meta["omega_bins"])
+    gpu.run(40)
+    for _ in range(40):
+        ref.step()
+        ref_mon.record(ref)
+    gpu.synchronize()
+    for name in FIELDS:
+        a, b = gpu.field(name), ref.field(name)
+        assert np.linalg.norm(b) > 0, name
+        d = np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b.astype(np.float64))
+        assert d < 1e-5, (name, d, int(np.count_nonzero(a != b)))
+    fa, fb = gpu.film_state(), ref.film_state()
+    assert fb["U_abs"].max() > 0
+    for k in ("Te", "Tl", "Ts_TM", "Ts_RE"):
+        d = np.linalg.norm(fa[k].astype(np.float64) - fb[k]) / np.linalg.norm(fb[k])
+        assert d < 1e-6, k
+    for k in ("m_TM_x", "m_TM_y", "m_TM_z", "m_RE_x", "m_RE_y", "m_RE_z"):
+        assert np.abs(fa[k].astype(np.float64) - fb[k]).max() < 1e-4, k
+    reached = 0
+    for p in range(len(meta["planes"])):
+        g, r = mon.download(p), ref_mon.download(p)
+        for k in ("ea", "eb", "ha", "hb", "energy"):
+            den = np.linalg.norm(r[k])
+            if den == 0:    # the z plane 45 cells below the source is not reached in 40 steps
+                assert not np.any(g[k]), (p, k)
+                continue
+            reached += 1
+            assert np.linalg.norm(g[k] - r[k]) / den < 1e-10, (p, k)
+    assert reached >= 20
+    mon.free()
+    gpu.free()
