@@ -131,18 +131,33 @@ def step_bytes(spec, planes=(), nb: int = 0, eps_index: bool = False, dft_batch:
 # on: `digest` is source_digest() of those files at the time, and a lookup whose current
 # digest differs returns nothing, so a figure from older kernels is never reported as one
 # of the running build.
-YEE_SOURCES = ("yee.hip", "common.h", "Makefile")
-FILM_SOURCES = ("thermo.hip", "film_physics.h", "common.h", "Makefile")
+YEE_SOURCES = ("yee.hip", "common.h")
+FILM_SOURCES = ("thermo.hip", "film_physics.h", "common.h")
 
 _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "csrc")
 
 
+def compile_flags() -> bytes:
+    """The compiler and flag definitions of csrc/Makefile (HIPCC, ARCH, HIPFLAGS with its
+    continuation lines): what, besides the sources, decides the kernels' code."""
+    out, cont = [], False
+    with open(os.path.join(_CSRC, "Makefile"), "rb") as f:
+        for line in f:
+            s = line.strip()
+            if cont or s.startswith((b"HIPCC", b"ARCH", b"HIPFLAGS")):
+                out.append(s)
+                cont = s.endswith(b"\\")
+    return b"\n".join(out)
+
+
 def source_digest(names) -> str:
-    """First 16 hex digits of the sha256 of the named csrc files, concatenated in order."""
+    """First 16 hex digits of the sha256 of the named csrc files, concatenated in order, and
+    the compile flags."""
     h = hashlib.sha256()
     for n in names:
         with open(os.path.join(_CSRC, n), "rb") as f:
             h.update(f.read())
+    h.update(compile_flags())
     return h.hexdigest()[:16]
 
 
@@ -150,14 +165,14 @@ def source_digest(names) -> str:
 # dispatch; the factors are what the counters read for a plain 1 GiB device copy on gfx950).
 MEASURED_TRAFFIC = {
     ("coupled_pump_512x512x256", "plain", "index"): {
-        "digest": "1da1f41030312908", "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md",
+        "digest": "f4b9e11c4dcd5ffc", "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md",
         "bytes": {"yee_E": 4133415722, "yee_H": 3222247005}},
 }
 
 # FP64 operations of one film cell-substep of the 4TM + LLB update from SQ_INSTS_VALU_*_F64
 # counters (2 x FMA + ADD + MUL, x 64 lanes, per dispatch / (cells x substeps)).
 FP64_OPS = {
-    "coupled_pump_512x512x256": {"digest": "cab2f6c1d5338377", "ops": 1143.4,
+    "coupled_pump_512x512x256": {"digest": "d7025c771f2d0f28", "ops": 1143.4,
                                  "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md"},
 }
 
