@@ -558,6 +558,8 @@ int dft_flush(const mp_yee *g, const mp_plane *planes, int32_t nplanes, int32_t 
     return MP_OK;
 }
 
+bool dft_batch_full() { return g_dft.count == dft_batch_size(); }
+
 int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes, const double *omega_bins,
                        double *phase, int32_t nb, double t, double dt) {
     const int K = dft_batch_size();
@@ -566,6 +568,9 @@ int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
     if (!g || (!planes && nplanes > 0)) return fail_arg("dft record: null argument");
     if (nplanes <= 0 || nb <= 0) return MP_OK;
     if (!omega_bins) return fail_arg("dft record: null bins");
+    // a full batch the caller has not flushed yet goes first
+    if (g_dft.count == K)
+        if (int e = dft_flush(g, planes, nplanes, nb, dt)) return e;
     PlaneSet ps;
     int64_t maxnp, total;
     if (int e = build_planes(g, planes, nplanes, ps, &maxnp, &total, "dft record")) return e;
@@ -597,7 +602,7 @@ int dft_record_batched(const mp_yee *g, const mp_plane *planes, int32_t nplanes,
                 g->nx, g->ny, slot, omega_bins, nb, t, dt);
         MP_LAUNCH_CHECK("dft_snapshot_kernel");
     }
-    if (++g_dft.count == K) return dft_flush(g, planes, nplanes, nb, dt);
+    ++g_dft.count;   // a full batch waits for dft_flush (the step driver runs it where it is cheapest)
     return MP_OK;
 }
 
