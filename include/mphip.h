@@ -289,7 +289,9 @@ typedef struct mp_state {
      * other, so the interior H and E updates run as one fused launch (73 instead of 85
      * bytes per interior cell).  Each pair ends with the fields back in the yee's volumes;
      * an odd last step runs in place.  The results are those of the in-place sequence bit
-     * for bit.  The library keeps no pointer after the call.  NULL keeps the in-place steps. */
+     * for bit.  The library keeps no pointer after the call.  NULL keeps the in-place steps.
+     * Should a launch fail inside a pair, the call returns its error with the fields of the
+     * step in progress possibly in this second set. */
     void *yee_alt[6];
 } mp_state;
 

@@ -240,7 +240,7 @@ MP_API int mp_run_steps(mp_state *s, int64_t nsteps) {
     }
     // DFT steps still held in the batch reach the accumulators before the call returns
     if (s->record_dft && s->nplanes > 0) {
-        if (int rc = dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt)) {
+        if ((rc = dft_flush(s->yee, s->planes, s->nplanes, s->nb, s->dt))) {
             overlap_join();
             return rc;
         }
