@@ -422,3 +422,25 @@ def accuracy_forms(kinds=("bf16", "f16"), tol: float = 1e-4, fluences=(35.0, 50.
     out["study_s"] = round(time.perf_counter() - t0, 1)
     sur.free()
     return out
+
+
+# ---- the C2 pump trajectories ------------------------------------------------------------------
+C2_SHAPE = (256, 256, 192)
+C2_FLUENCES = (2.0, 4.0, 6.0, 8.0, 10.0, 12.0, 14.0, 16.0)
+
+
+def c2_rows(fluences=C2_FLUENCES, steps: int = 4000, every: int = 100, stride: int = 8):
+    """Training rows from plain runs of the 256 x 256 x 192 coupled pump at each fluence
+    (mJ/cm^2): `steps` steps with the pulse peak in the middle, every `every`-th step the film
+    state before that step's film update, every `stride`-th film cell."""
+    from . import configs
+    nx, ny, nz = C2_SHAPE
+    Xs, Ts = [], []
+    for fl in fluences:
+        spec, _ = configs.coupled_pump(nx=nx, ny=ny, nz=nz, steps=steps, fluence_mj_cm2=float(fl), n_planes=0)
+        st = FDTDState(spec)
+        X, T = collect_trajectory(st, steps, every, stride=stride)
+        st.free()
+        Xs.append(X)
+        Ts.append(T)
+    return np.concatenate(Xs), np.concatenate(Ts)

@@ -225,3 +225,57 @@ def test_trained_surrogate_gain_on_relax():
     assert res["surrogate_tol0.0001"]["film_mean_dm"] < 5e-3
     assert res["surrogate_tol0.001"]["fallback"] < 0.5
     sur.free()
+
+
+def test_c4_surrogate_trained_on_c2_trajectories():
+    """The learned film update as the workload table sets it up: trained on the GPU on about
+    2.6e7 rows of plain 256 x 256 x 192 pump runs at 2-16 mJ/cm^2 (every 100th of 4000 steps,
+    every 8th film cell), then run on the 2.62 M-cell film of the 512 x 512 x 256 pump at
+    12 mJ/cm^2 for 4000 steps (plain fallback) against the plain integrator from the same
+    start, at check tolerances 1e-4 and 1e-5.  Measured: Te peaks at 1317 K; at 1e-5 the
+    final film-mean |dm| is 5.5e-4 (the 1e-3 bar holds, 0.37 % fallback), at 1e-4 it is
+    1.05e-3 (0.08 % fallback): the per-step check accumulates over 4000 steps."""
+    import time
+    from mphip import configs
+    from mphip import surrogate as S
+    from mphip.state import FDTDState
+    print("case C4: surrogate trained on C2 trajectories, run on the 512x512x256 film", flush=True)
+    t0 = time.perf_counter()
+    X, T = S.c2_rows()
+    X, T = S.shuffled(X, T, 0)
+    sur = S.Surrogate(norm=S.make_norm(X[:200000], T[:200000]), seed=1)
+    data = S.Dataset(X, T)
+    sur.train(data, 10000, batch=65536, lr=3e-3, lr_end=1e-5)
+    loss = sur.evaluate(data, rows=262144)
+    data.free()
+    print(f"rows {X.shape[0]}  train loss {loss:.3g}  {time.perf_counter() - t0:.1f} s", flush=True)
+    assert X.shape[0] > 2e7
+    del X, T
+    spec, _ = configs.coupled_pump(nx=512, ny=512, nz=256, steps=4000, fluence_mj_cm2=12.0, n_planes=0)
+    films, fbs = {}, {}
+    for name, tol in (("plain", None), ("tol1e-4", 1e-4), ("tol1e-5", 1e-5)):
+        st = FDTDState(spec)
+        if tol is not None:
+            st.attach_surrogate(sur, tol)
+            sur.reset_fallbacks()
+        t1 = time.perf_counter()
+        st.run(4000)
+        st.synchronize()
+        ms = (time.perf_counter() - t1) / 4000 * 1e3
+        films[name] = st.film_state()
+        fbs[name] = sur.fallback_count() / (spec.n_film * 4000) if tol is not None else 0.0
+        st.free()
+        print(f"{name}: {ms:.3f} ms/step", flush=True)
+    b = films["plain"]
+    assert b["Te"].max() > 310.0       # the film is driven
+    res = {}
+    for name in ("tol1e-4", "tol1e-5"):
+        a = films[name]
+        dm = S.film_mean_dm(a, b)
+        te = float(np.linalg.norm(a["Te"].astype(np.float64) - b["Te"]) / np.linalg.norm(b["Te"].astype(np.float64)))
+        res[name] = (dm, te, fbs[name])
+        print(f"{name}: Te max {float(b['Te'].max()):.1f} K  film-mean |dm| {dm:.3g}  Te rel L2 {te:.3g}  "
+              f"fallback {fbs[name]:.4f}", flush=True)
+    assert res["tol1e-5"][0] <= 1e-3 and res["tol1e-5"][1] < 1e-3
+    assert res["tol1e-4"][0] < 2e-3 and res["tol1e-4"][2] < 0.01
+    sur.free()
