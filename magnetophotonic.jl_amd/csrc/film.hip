@@ -355,6 +355,182 @@ __global__ __launch_bounds__(256) void film_epilogue_kernel(
     if (pabs > peak[n]) peak[n] = pabs;
 }
 
+// The same per-cell operations with NP poles in every ADE list and in the MO list (the film's
+// Drude + Lorentz ADE poles and its two probe MO poles: NP = 2), every load of the cell issued
+// before its first store.  In film_epilogue_kernel a store through one list's arrays might
+// alias a later load through another list's, so each list's loads wait behind the previous
+// list's stores: one memory round trip per list.  The arithmetic and its order are unchanged.
+// Every array is addressed by a 32-bit byte offset from its base in scalar registers (the
+// launcher checks that the offsets fit), so the cell's loads share a few offset registers.
+template <typename T> __device__ __forceinline__ T &bref(T *base, uint32_t boff) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff);
+}
+template <typename T> __device__ __forceinline__ const T &bref(const T *base, uint32_t boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <typename C, int NP> struct AdeCell {
+    C Eprev, f, ie, P[NP], J[NP];
+};
+template <typename C, int NP> struct MoCell {
+    C f, m_TM, m_RE, Eoa, Eob, Pa[NP], Ja[NP], Pb[NP], Jb[NP];
+};
+// oc: byte offset of entry n in a C array; ot: of entry n in a T array; stT: of one pole's
+// block (N entries) in a T array
+template <typename C, typename T, int NP>
+__device__ __forceinline__ AdeCell<C, NP> ade_cell_load(const AdeView<C, T> &a, uint32_t oc, uint32_t ot,
+                                                        uint32_t stT) {
+    AdeCell<C, NP> c;
+    c.Eprev = bref(a.E_old, oc);
+    c.f = bref(a.fill, oc);
+    c.ie = bref(a.inv_eps, oc);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        c.P[p] = StateIO<C, T>::dec(bref(a.P, ot + p * stT), a.isc);
+        c.J[p] = StateIO<C, T>::dec(bref(a.J, ot + p * stT), a.ijsc);
+    }
+    return c;
+}
+// ade_entry's arithmetic on loaded values
+template <typename S, typename C, typename T, int NP>
+__device__ __forceinline__ void ade_cell_apply(S &Ecell, const AdeView<C, T> &a, const AdeCell<C, NP> &c,
+                                               const C *__restrict__ poles, uint32_t oc, uint32_t ot,
+                                               uint32_t stT, C inv_alpha, double &jsum) {
+    C sum_psi = (C)0, sum_beta = (C)0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+        const C beta = poles[3 * p + 2] * c.f;
+        const C psi = fmaT<C>(poles[3 * p], c.P[p], fmaT<C>(poles[3 * p + 1], c.J[p], beta * c.Eprev));
+        sum_psi += psi;
+        sum_beta += beta;
+    }
+    const C den = (C)1 + sum_beta * c.ie;
+    const C Etemp = (C)Ecell;
+    jsum = 0.0;
+    if (finiteT(Etemp) && finiteT(sum_psi) && finiteT(den) && __builtin_fabs((double)den) >= (double)(C)1e-30) {
+        const C Efinal = (Etemp - sum_psi * c.ie) / den;
+        Ecell = (S)Efinal;
+        bref(a.E_old, oc) = Efinal;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const C beta = poles[3 * p + 2] * c.f;
+            const C psi = fmaT<C>(poles[3 * p], c.P[p], fmaT<C>(poles[3 * p + 1], c.J[p], beta * c.Eprev));
+            const C Pnew = psi + beta * Efinal;
+            const C Jnew = (Pnew - c.P[p]) * inv_alpha - c.J[p];
+            bref(a.P, ot + p * stT) = StateIO<C, T>::enc(Pnew, a.sc);
+            const T jq = StateIO<C, T>::enc(Jnew, a.jsc);
+            bref(a.J, ot + p * stT) = jq;
+            jsum += (double)StateIO<C, T>::dec(jq, a.ijsc);
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < NP; ++p) jsum += (double)c.J[p];
+    }
+}
+
+template <typename S, typename C, typename T, bool HAS_MO, bool ADE_WORK, int NP>
+__global__ __launch_bounds__(256) void film_epilogue_np_kernel(
+    S *__restrict__ Ex, S *__restrict__ Ey, S *__restrict__ Ez, const int32_t *__restrict__ cells,
+    AdeView<C, T> ax, AdeView<C, T> ay, AdeView<C, T> az, const C *__restrict__ poles, int, C inv_alpha,
+    MoView<C> mo, const C *__restrict__ mo_poles, int, int mo_a, int mo_b,
+    const S *__restrict__ iea, const S *__restrict__ ieb, C Q_TM, C Q_RE, C mo_inv_alpha,
+    double *__restrict__ U_abs, double *__restrict__ window, double *__restrict__ peak, double pabs_factor,
+    double dt, int64_t N) {
+    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (uint64_t)N) return;
+    const uint32_t oc = n * (uint32_t)sizeof(C), ot = n * (uint32_t)sizeof(T), od = n * 8u;
+    const uint32_t stC = (uint32_t)N * (uint32_t)sizeof(C), stT = (uint32_t)N * (uint32_t)sizeof(T);
+    // every load of the cell
+    const uint32_t oe = (uint32_t)bref(cells, n * 4u) * (uint32_t)sizeof(S);
+    S e[3] = {bref(Ex, oe), bref(Ey, oe), bref(Ez, oe)};
+    const AdeCell<C, NP> cx = ade_cell_load<C, T, NP>(ax, oc, ot, stT);
+    const AdeCell<C, NP> cy = ade_cell_load<C, T, NP>(ay, oc, ot, stT);
+    const AdeCell<C, NP> cz = ade_cell_load<C, T, NP>(az, oc, ot, stT);
+    MoCell<C, NP> m{};
+    C ica = (C)0, icb = (C)0;
+    if (HAS_MO) {
+        m.f = bref(mo.fill, oc);
+        m.m_TM = bref(mo.m_TM, oc);
+        m.m_RE = bref(mo.m_RE, oc);
+        m.Eoa = bref(mo.Eoa, oc);
+        m.Eob = bref(mo.Eob, oc);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            m.Pa[p] = bref(mo.Pa, oc + p * stC);
+            m.Ja[p] = bref(mo.Ja, oc + p * stC);
+            m.Pb[p] = bref(mo.Pb, oc + p * stC);
+            m.Jb[p] = bref(mo.Jb, oc + p * stC);
+        }
+        ica = (C)bref(iea, oe);
+        icb = (C)bref(ieb, oe);
+    }
+    const double U_old = bref(U_abs, od), W_old = bref(window, od), peak_old = bref(peak, od);
+    // ade_patch_kernel x3
+    double js[3];
+    ade_cell_apply<S, C, T, NP>(e[0], ax, cx, poles, oc, ot, stT, inv_alpha, js[0]);
+    ade_cell_apply<S, C, T, NP>(e[1], ay, cy, poles, oc, ot, stT, inv_alpha, js[1]);
+    ade_cell_apply<S, C, T, NP>(e[2], az, cz, poles, oc, ot, stT, inv_alpha, js[2]);
+    if (HAS_MO) {
+        // mo_gyration_kernel on entry n
+        const C f = m.f;
+        const C qeff = Q_TM * m.m_TM + Q_RE * m.m_RE;
+        const S ea_s = mo_a == 0 ? e[0] : (mo_a == 1 ? e[1] : e[2]);
+        const S eb_s = mo_b == 0 ? e[0] : (mo_b == 1 ? e[1] : e[2]);
+        const C Ea_drive = (C)ea_s;
+        const C Eb_drive = (C)eb_s;
+        const C Ea_old = m.Eoa;
+        const C Eb_old = m.Eob;
+        C sum_psi_a = (C)0, sum_psi_b = (C)0, sumC3 = (C)0;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const C C1 = mo_poles[3 * p], C2 = mo_poles[3 * p + 1], C3 = mo_poles[3 * p + 2];
+            const C beta = C3 * f * qeff;
+            sum_psi_a += fmaT<C>(C1, m.Pa[p], fmaT<C>(C2, m.Ja[p], beta * Eb_old));
+            sum_psi_b += fmaT<C>(C1, m.Pb[p], fmaT<C>(C2, m.Jb[p], beta * Ea_old));
+            sumC3 += C3;
+        }
+        const C Bmo = f * qeff * sumC3;
+        const C half = (C)0.5;
+        const C aa = half * ica * Bmo;
+        const C ab = half * icb * Bmo;
+        const C RHSa = Ea_drive - ica * sum_psi_a - aa * Eb_drive;
+        const C RHSb = Eb_drive + icb * sum_psi_b + ab * Ea_drive;
+        const C den = (C)1 + aa * ab;
+        const C Eb_final = (RHSb + ab * RHSa) / den;
+        const C Ea_final = RHSa - aa * Eb_final;
+        if (finiteT(Ea_final) && finiteT(Eb_final)) {
+            const S ea_new = (S)Ea_final, eb_new = (S)Eb_final;
+            if (mo_a == 0) e[0] = ea_new; else if (mo_a == 1) e[1] = ea_new; else e[2] = ea_new;
+            if (mo_b == 0) e[0] = eb_new; else if (mo_b == 1) e[1] = eb_new; else e[2] = eb_new;
+            bref(mo.Eoa, oc) = Ea_final;
+            bref(mo.Eob, oc) = Eb_final;
+            const C Ea_mid = half * (Ea_drive + Ea_final);
+            const C Eb_mid = half * (Eb_drive + Eb_final);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) {
+                const C C1 = mo_poles[3 * p], C2 = mo_poles[3 * p + 1], C3 = mo_poles[3 * p + 2];
+                const C beta = C3 * f * qeff;
+                const C Pa_new = fmaT<C>(C1, m.Pa[p], fmaT<C>(C2, m.Ja[p], beta * Eb_old)) + beta * Eb_mid;
+                bref(mo.Pa, oc + p * stC) = Pa_new;
+                bref(mo.Ja, oc + p * stC) = (Pa_new - m.Pa[p]) * mo_inv_alpha - m.Ja[p];
+                const C Pb_new = fmaT<C>(C1, m.Pb[p], fmaT<C>(C2, m.Jb[p], beta * Ea_old)) + beta * Ea_mid;
+                bref(mo.Pb, oc + p * stC) = Pb_new;
+                bref(mo.Jb, oc + p * stC) = (Pb_new - m.Pb[p]) * mo_inv_alpha - m.Jb[p];
+            }
+        }
+    }
+    bref(Ex, oe) = e[0];
+    bref(Ey, oe) = e[1];
+    bref(Ez, oe) = e[2];
+    // pabs_kernel on film cell n
+    const double ex = (double)e[0], ey = (double)e[1], ez = (double)e[2];
+    const double pabs = ADE_WORK ? jl_max(ex * js[0] + ey * js[1] + ez * js[2], 0.0)
+                                 : pabs_factor * (ex * ex + ey * ey + ez * ez);
+    const double pdt = pabs * dt;
+    bref(U_abs, od) = U_old + pdt;
+    bref(window, od) = W_old + pdt;
+    if (pabs > peak_old) bref(peak, od) = pabs;
+}
+
 template <typename S, typename C, typename T>
 static int launch_epilogue(const mp_yee *g, mp_ade *const ade[3], const mp_mo *mo, int mo_a, int mo_b,
                            const void *iea, const void *ieb, mp_film *f, int has_ade_work, double pabs_factor,
@@ -383,7 +559,20 @@ static int launch_epilogue(const mp_yee *g, mp_ade *const ade[3], const mp_mo *m
                 (const S *)iea, (const S *)ieb, (C)Q_TM, (C)Q_RE, mo_inv_alpha, f->U_abs, f->window_sum, f->peak, \
                 pabs_factor, dt, N
     const unsigned grid = cdiv(N, 256);
-    if (mo) {
+    // the two-pole kernel addresses every array by a 32-bit byte offset: the E volumes and
+    // two pole blocks of the widest per-cell array (8-byte U_abs / window / peak) must fit
+    const int64_t cells_total = g->nx * g->ny * g->nz;
+    const bool offsets_fit = cells_total * (int64_t)sizeof(S) < ((int64_t)1 << 32) &&
+                             2 * N * (int64_t)std::max<size_t>(sizeof(C), 8) < ((int64_t)1 << 32);
+    if (ade[0]->npoles == 2 && (!mo || mo->npoles == 2) && offsets_fit) {
+        if (mo) {
+            if (has_ade_work) film_epilogue_np_kernel<S, C, T, true, true, 2><<<grid, 256, 0, stream()>>>(EP_ARGS);
+            else film_epilogue_np_kernel<S, C, T, true, false, 2><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        } else {
+            if (has_ade_work) film_epilogue_np_kernel<S, C, T, false, true, 2><<<grid, 256, 0, stream()>>>(EP_ARGS);
+            else film_epilogue_np_kernel<S, C, T, false, false, 2><<<grid, 256, 0, stream()>>>(EP_ARGS);
+        }
+    } else if (mo) {
         if (has_ade_work) film_epilogue_kernel<S, C, T, true, true><<<grid, 256, 0, stream()>>>(EP_ARGS);
         else film_epilogue_kernel<S, C, T, true, false><<<grid, 256, 0, stream()>>>(EP_ARGS);
     } else {

@@ -344,6 +344,9 @@ struct ShellGeom {
     uint32_t G, ny, nz, gi0, ngi, my, nj, mz;
     uint32_t pfull, pint, gx, nlo, nmid;   // groups per whole plane / per box plane, strip groups per row
     uint32_t r_pfull, r_pint, r_g, r_gx;   // ceil(2^32 / d) of those divisors
+    uint32_t sgx;                        // strip groups per box row in this layout: gx, or 0 when the
+                                         // x strips have a launch of their own
+    uint32_t nstrip, r_nj;               // groups of all x strips; ceil(2^32 / nj)
 };
 __device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t d, uint32_t r) {
     uint32_t q = __umulhi(n, r);
@@ -363,7 +366,7 @@ __device__ __forceinline__ void shell_pos(const ShellGeom &sg, uint32_t t, uint3
     const uint32_t kk = udiv(u, sg.pint, sg.r_pint);
     uint32_t r = u - kk * sg.pint;
     k = sg.mz + kk;
-    const uint32_t lo = sg.my * sg.G, strips = sg.nj * sg.gx;
+    const uint32_t lo = sg.my * sg.G, strips = sg.nj * sg.sgx;
     if (r < lo || r >= lo + strips) {   // a whole low or high row
         const bool hi = r >= lo;
         if (hi) r -= lo + strips;
@@ -376,6 +379,17 @@ __device__ __forceinline__ void shell_pos(const ShellGeom &sg, uint32_t t, uint3
         j = sg.my + jj;
         g = gg < sg.gi0 ? gg : gg + sg.ngi;
     }
+}
+
+// X-strip entry t of a box (groups [0, gi0) and [gi0+ngi, G) of the nj box rows of the nk box
+// planes, row after row): every such group lies outside the y and z slabs and off the high
+// boundary rows and planes.
+__device__ __forceinline__ void strip_pos(const ShellGeom &sg, uint32_t t, uint32_t &g, uint32_t &j, uint32_t &k) {
+    const uint32_t r = udiv(t, sg.gx, sg.r_gx), s = t - r * sg.gx;
+    const uint32_t kk = udiv(r, sg.nj, sg.r_nj);
+    j = sg.my + (r - kk * sg.nj);
+    k = sg.mz + kk;
+    g = s < sg.gi0 ? s : s + sg.ngi;
 }
 
 // The group a lane of a vector kernel updates: either the flattened plane mapping above
@@ -415,7 +429,10 @@ __device__ __forceinline__ bool lane_holds(int64_t o, int delta) {
 
 // PP: the updated H goes to Hxo/Hyo/Hzo (the other buffer set of a ping-pong step) instead of
 // back to Hx/Hy/Hz; the whole group is stored, so a cell at i = nx-1 carries its old value over.
-template <typename P, int TYV, bool PP>
+// STRIP: the lanes take the x strips of the box rows (strip_pos), where no y or z slab term
+// exists: those terms and their registers drop out, and the kernel fits two waves per SIMD
+// beside the film update, where the general form fits one.
+template <typename P, int TYV, bool PP, bool STRIP = false>
 __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
@@ -423,7 +440,18 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, ShellGeom sg,
     float *__restrict__ Hxo, float *__restrict__ Hyo, float *__restrict__ Hzo) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
-    const LaneGroup lg = lane_group(sg, nx, ny, nbx, TYV);
+    LaneGroup lg;
+    if (STRIP) {
+        const uint32_t t = blockIdx.x * (uint32_t)(64 * TYV) + threadIdx.y * 64 + threadIdx.x;
+        lg.in_range = t < sg.nstrip;
+        uint32_t g, j, k;
+        strip_pos(sg, lg.in_range ? t : 0u, g, j, k);
+        lg.i0 = (int64_t)g * VX;
+        lg.j = (int64_t)j;
+        lg.k = (int64_t)k;
+    } else {
+        lg = lane_group(sg, nx, ny, nbx, TYV);
+    }
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
     const bool act = lg.in_range && lg.j < ny - 1 && lg.k < nz - 1;
     const int64_t k = act ? lg.k : 0;
@@ -435,8 +463,8 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const int64_t wx = cp.w[0], wy = cp.w[1], wz = cp.w[2];
     const int64_t ic = act ? lg.i0 : 0;   // clamp addresses of idle lanes
     const int64_t jc = act ? lg.j : (int64_t)0;   // rows 0 and 1 exist (ny >= 2)
-    int64_t jsy;
-    const int ry = pml_slab(jc, ny, wy, &jsy);
+    int64_t jsy = 0;
+    const int ry = STRIP ? 0 : pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(idcx + ic), xk4 = ld4(xk + ic);
     const float idy_ = idcy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
     const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
@@ -453,8 +481,8 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
         ey_nx = Ey[o + VX];
     }
     const float idz_ = idcz[k], zk_ = zk[k];
-    int64_t jsz;
-    const int rz = pml_slab(k, nz, wz, &jsz);
+    int64_t jsz = 0;
+    const int rz = STRIP ? 0 : pml_slab(k, nz, wz, &jsz);
     const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
     // y-slab psi of Hx and Hz, z-slab psi of Hx and Hy (lo or hi slab array)
     const int64_t oy0 = ic + nx * (jsy + wy * k), oz0 = ic + nx * (jc + ny * jsz);
@@ -1123,12 +1151,17 @@ static bool split_ok(const mp_yee *g, const Box &b) {
 
 // Shell layout of a box (see ShellGeom); n = 0 when a divisor would be below 2, where the
 // launchers keep the whole-volume launch.
-static ShellGeom shell_geom(const mp_yee *g, const Box &b) {
+// strips_apart: the x strips of the box rows are left out of the layout (sgx = 0) for a
+// launch of their own (nstrip groups).
+static ShellGeom shell_geom(const mp_yee *g, const Box &b, bool strips_apart = false) {
     ShellGeom sg{};
     const int64_t G = g->nx / VX, ny = g->ny, nz = g->nz;
-    const int64_t pfull = ny * G, gx = G - b.ngi, pint = 2 * b.j0 * G + b.nj * gx;
+    const int64_t pfull = ny * G, gx = G - b.ngi, sgx = strips_apart ? 0 : gx;
+    const int64_t pint = 2 * b.j0 * G + b.nj * sgx, nstrip = b.nj * b.nk * gx;
     const int64_t nlo = b.k0 * pfull, nmid = b.nk * pint, n = 2 * nlo + nmid;
-    if (pfull < 2 || pint < 2 || G < 2 || gx < 2 || n != G * ny * nz - b.n()) return sg;
+    if (pfull < 2 || pint < 2 || G < 2 || gx < 2 || (strips_apart && b.nj < 2) ||
+        n + (strips_apart ? nstrip : 0) != G * ny * nz - b.n())
+        return sg;
     auto recip = [](int64_t d) { return (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d); };
     sg.n = (uint32_t)n;
     sg.G = (uint32_t)G;
@@ -1148,6 +1181,9 @@ static ShellGeom shell_geom(const mp_yee *g, const Box &b) {
     sg.r_pint = recip(pint);
     sg.r_g = recip(G);
     sg.r_gx = recip(gx);
+    sg.sgx = (uint32_t)sgx;
+    sg.nstrip = (uint32_t)nstrip;
+    sg.r_nj = recip(b.nj);
     return sg;
 }
 
@@ -1192,12 +1228,23 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         (const float *)c->inv_kappa[2], nx, ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0,
         (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0, (uint32_t)b.n(), dt_mu0);
     MP_LAUNCH_CHECK("yee_H_int_kernel");
-    yee_H_vec_kernel<P, TY, false><<<cdiv(sg.n, 64 * TY), block, 0, stream()>>>(
+    // the x strips of the box rows in a launch of their own, the rest of the shell in another
+    const ShellGeom ss = shell_geom(g, b, true);
+    const ShellGeom &sr = ss.n ? ss : sg;
+    yee_H_vec_kernel<P, TY, false><<<cdiv(sr.n, 64 * TY), block, 0, stream()>>>(
         (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
         (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sg, nullptr, nullptr,
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sr, nullptr, nullptr,
         nullptr);
     MP_LAUNCH_CHECK("yee_H_vec_kernel");
+    if (ss.n) {
+        yee_H_vec_kernel<P, TY, false, true><<<cdiv(ss.nstrip, 64 * TY), block, 0, stream()>>>(
+            (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
+            (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
+            (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, ss, nullptr, nullptr,
+            nullptr);
+        MP_LAUNCH_CHECK("yee_H_vec_kernel");
+    }
     return MP_OK;
 }
 

@@ -338,14 +338,20 @@ def test_batched_dft_record_is_bitwise_per_step():
     b.free()
 
 
-def test_fused_film_epilogue_matches_separate_kernels():
+@pytest.mark.parametrize("poles,dtype", [(2, "f32"), (3, "f32"), (2, "f64")])
+def test_fused_film_epilogue_matches_separate_kernels(poles, dtype):
     """The one-pass film list work (ADE x/y/z, gyration, absorbed power) against the five
-    separate launches on the same state: every field, list state and film value equal."""
-    from mphip import configs
+    separate launches on the same state: every field, list state and film value equal.  With
+    the film's two ADE and two MO poles the pass runs its two-pole kernel (every load of a cell
+    before its first store); with a third ADE pole, the kernel for any pole count."""
+    from mphip import configs, dispersion, grid
     from mphip.state import FDTDState
     spec, meta = configs.coupled_pump(nx=40, ny=36, nz=64, steps=150, fluence_mj_cm2=12.0, npml=6,
-                                       src_z=40e-9)
-    print("case coupled_pump 40x36x64 fused vs separate, 150 steps", flush=True)
+                                       src_z=40e-9, dtype=dtype)
+    if poles == 3:
+        extra = dispersion.create_pole(6e15, 8e14, 0.5 * (6e15) ** 2, spec.dt, grid.EPS0)
+        spec.diag_poles = dispersion.poles_array(configs.film_poles(spec.dt) + [extra], spec.compute_dtype)
+    print(f"case coupled_pump 40x36x64 fused vs separate, {poles} ADE poles, {dtype}, 150 steps", flush=True)
     fused, sep = FDTDState(spec), FDTDState(spec)
     assert fused.fused_epilogue, "the pump's lists are the film list"
     sep.fused_epilogue = False
