@@ -8,8 +8,8 @@ A step is one pass of the full coupled sequence (Yee H, pump source, Yee E, ADE 
 gyrotropic MO, absorbed power, 4TM + LLB with 4 substeps, DFT on 6 planes x 4 bins)
 over the 512 x 512 x 256 synthetic pump workload, inputs resident in HBM.  Prints one
 JSON line: Mcell-updates/s = Nx*Ny*Nz*K / wall, the roofline of the dominant kernel
-(algorithmic bytes per launch / mean launch time from HIP events recorded around each of
-its launches in the timed region; the per-kernel breakdown comes from further steps with
+(algorithmic bytes per launch / mean launch time from HIP events recorded around its
+launches in every 4th step of the timed region; the per-kernel breakdown comes from further steps with
 every launch bracketed, then from as many steps with the film update serial, each kernel
 alone), the FP64 roofline of the film update, and the CPU oracle timed on the host cores
 (FP32 on every core, FP32 on one thread, FP64 on every core).  Counter figures (measured
@@ -43,6 +43,7 @@ KERNELS = ("yee_H", "yee_HE", "yee_E", "pingpong_sync", "inject_plane", "ade_pat
            "film_epilogue", "multiphysics", "surrogate", "surrogate_fallback", "dft_phase", "dft_accumulate",
            "dft_snapshot", "dft_batch")
 DFT_BATCH = 32   # steps per batched DFT pass in the library
+PROFILE_EVERY = 4   # timed steps per bracketed launch of the dominant kernel
 
 
 FORMS = ("plain", "mixed", "surrogate", "surrogate+mixed")
@@ -203,6 +204,9 @@ def main():
     ap.add_argument("--compare-steps", type=int, default=50, help="timed steps per form in the form comparison")
     ap.add_argument("--pingpong", action="store_true",
                     help="ping-pong steps: a second E/H buffer set and the fused interior H + E launch")
+    ap.add_argument("--film-serial", action="store_true",
+                    help="the film update on the launch stream after each step's other launches "
+                         "(one stream; for counter runs, whose profiler serialises every launch)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -238,6 +242,8 @@ def main():
     if surrogate_form:
         sur, sur_info = train_pump_surrogate(args.warmup + args.steps)
         st.attach_surrogate(sur, args.tol)
+    if args.film_serial:
+        st.film_serial = True
     st.run(args.warmup)
     st.synchronize()
     if surrogate_form:
@@ -254,8 +260,11 @@ def main():
     dom = max((k for k in kb if k in KERNELS), key=lambda k: kb[k])
 
     # timed region: K steps; HIP events on the launch stream bracket the dominant kernel's
-    # launches only, so no other launch waits behind an event record
+    # launches only, and only in every PROFILE_EVERY-th step: each bracket is two event
+    # records, which hold the launch stream ~12 us (the next launch waits for each record)
     L.mp_profile_select(dom.encode())
+    every = PROFILE_EVERY if args.steps >= 2 * PROFILE_EVERY else 1
+    L.mp_profile_sample(every)
     L.mp_profile_enable(1)
     st.synchronize()
     t0 = time.perf_counter()
@@ -272,6 +281,7 @@ def main():
 
     # per-kernel breakdown: further steps after the timed region, every launch bracketed
     brk_steps = min(args.steps, 50)
+    L.mp_profile_sample(1)
     L.mp_profile_select(None)
     L.mp_profile_enable(1)
     st.run(brk_steps)
@@ -286,7 +296,7 @@ def main():
     # the same steps with the film update on the main stream after the step's launches:
     # every kernel's time alone, without the side stream's company
     serial = {}
-    if spec.has_film:
+    if spec.has_film and not args.film_serial:
         st.film_serial = True
         L.mp_profile_select(None)
         L.mp_profile_enable(1)
@@ -300,7 +310,9 @@ def main():
         L.mp_profile_enable(0)
         st.film_serial = False
     mcell = ncell * args.steps / wall / 1e6
-    per_launch = kb[dom] / (dom_launches / args.steps)
+    # calls of the dominant kernel per step (one for a Yee update), from the sampled count
+    calls_per_step = max(1, round(dom_launches * every / args.steps))
+    per_launch = kb[dom] / calls_per_step
     achieved = per_launch / (dom_avg_us * 1e-6) / 1e9
     for k, v in stats.items():
         if k in kb:
@@ -347,8 +359,10 @@ def main():
                    "inv_eps": f"uint8 index, {st.yee.n_eps} triples" if eps_idx else "FP32 volumes",
                    "film_list_work": "one fused pass" if st.fused_epilogue else "five launches",
                    "yee_steps": "ping-pong pairs, fused interior H + E" if args.pingpong else "in place",
+                   "film_update": "serial, on the launch stream" if args.film_serial else "side stream, beside the next step's Yee updates",
                    "dft_batch": DFT_BATCH},
-        "roofline": {"bound": "hbm", "kernel": dom, "launches": dom_launches, "avg_us": round(dom_avg_us, 2),
+        "roofline": {"bound": "hbm", "kernel": dom, "launches": dom_launches, "timed_every": every,
+                     "avg_us": round(dom_avg_us, 2),
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_from": traffic_src,

@@ -335,6 +335,11 @@ int mp_kernel_stats(const char *name, int64_t *launches, double *total_ms);
  * times every kernel), so that a timed run brackets only the launches it reports.  No
  * counterpart in the reference (instrumentation of this library). */
 int mp_profile_select(const char *names);
+/* Time only every `every`-th launch of each timed kernel name, counted from this call
+ * (1, the default, times every launch).  Each bracket costs the launch stream two event
+ * records, so a timed run can sample its launches instead of delaying each step.  No
+ * counterpart in the reference (instrumentation of this library). */
+int mp_profile_sample(int32_t every);
 
 /* ---- hot-path kernels ---------------------------------------------------------------- */
 

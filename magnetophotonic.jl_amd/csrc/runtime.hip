@@ -18,6 +18,8 @@ static thread_local char g_err[512] = {0};
 static hipStream_t g_stream = nullptr;
 static bool g_profile = false;
 static std::vector<std::string> g_profile_only;   // empty: every kernel name is timed
+static int g_profile_every = 1;                    // time every n-th launch of each name
+static std::map<std::string, int64_t> g_profile_seen;
 
 struct PendingTiming {
     const char *name;
@@ -91,6 +93,7 @@ static bool profiled(const char *name) {
 
 KernelTimer::KernelTimer(const char *name) : name_(name) {
     if (!g_profile || !profiled(name)) return;
+    if (g_profile_every > 1 && g_profile_seen[name]++ % g_profile_every != 0) return;
     hipEvent_t e = pool_event();
     if (e && hipEventRecord(e, g_stream) == hipSuccess) start_ = (void *)e;
 }
@@ -244,6 +247,14 @@ MP_API int mp_profile_select(const char *names) {
             cur.push_back(*c);
         }
     }
+    return MP_OK;
+}
+
+MP_API int mp_profile_sample(int32_t every) {
+    if (every < 1) return fail_arg("mp_profile_sample: every must be >= 1 (got %d)", every);
+    drain_pending();
+    g_profile_every = every;
+    g_profile_seen.clear();
     return MP_OK;
 }
 

@@ -172,6 +172,7 @@ RUNTIME_SIGNATURES = {
     "mp_profile_enable": (C.c_int, [C.c_int]),
     "mp_kernel_stats": (C.c_int, [C.c_char_p, P(i64), P(f64)]),
     "mp_profile_select": (C.c_int, [C.c_char_p]),
+    "mp_profile_sample": (C.c_int, [C.c_int32]),
     "mp_any_nonfinite": (C.c_int, [vp, i32, i64, vp]),
 }
 
