@@ -519,14 +519,15 @@ int mp_surrogate_dataset(const mp_film *f, const mp_ferri_params *p, const mp_lu
                          int64_t stride, float *X, float *T, int64_t max_rows);
 
 /* Film update by the surrogate, in place of mp_multiphysics_step (same arguments).  Per
- * cell: the network's rates give the end state over dt (m_y, m_z extrapolated from one
- * plain substep of h = dt/subcycles).  The step is checked by the trapezoid rule: with
- * r0 the plain one-substep rate at the start and r1 the one at the network's end state,
- * the change must equal dt/2 (r0 + r1) to within tol (absolute in m, relative in T).
- * A cell that fails falls back to the plain integrator and finishes the remaining
- * substeps from its first one, so its result is the plain one.  Cost: the network and
- * 2 substeps per accepted cell.  tol < 0 forces the plain integrator everywhere.
- * *fallbacks (device int64, may be NULL) is incremented by the number of fallbacks. */
+ * cell: the network's rates give the end state over dt.  The step is checked against one
+ * plain substep of h = dt/subcycles taken at its midpoint (temperatures and m_x halfway
+ * along the predicted step, m_y and m_z at their start values): with r_mid that
+ * substep's rate, the change must equal dt r_mid to within tol (absolute in m, relative
+ * in T), the midpoint rule; m_y and m_z advance by dt r_mid.  A cell that fails falls back
+ * to the plain integrator from its start state, so its result is the plain one.  Cost:
+ * the network and 1 substep per accepted cell, subcycles more per fallback.  tol < 0
+ * forces the plain integrator everywhere.  *fallbacks (device int64, may be NULL) is
+ * incremented by the number of fallbacks. */
 int mp_surrogate_step(mp_film *f, const mp_ferri_params *p, const mp_lut *lut,
                       const mp_mlp *m, double dt, int32_t subcycles, int32_t brillouin_iters,
                       int32_t has_pabs, double tol, int64_t *fallbacks);

@@ -25,6 +25,10 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int OW1 = 0, OB1 = 1024, OW2 = 1088, OB2 = 5184, OW3 = 5248, OB3 = 6272;
 constexpr int NPARAM = MP_MLP_NPARAM;
+// blocks of the surrogate film update (4 waves each): one wave per SIMD on 256 CUs
+#ifndef SUR_BLOCKS
+#define SUR_BLOCKS 256
+#endif
 // fragment index ranges inside the packed buffer (64 lanes x uint2 each)
 constexpr int F_W1 = 0, F_W2 = 4, F_W3 = 20, F_W3T = 24, F_W2T = 28, NFRAG = 44, NFWD = 24;
 static_assert(NFRAG * 64 * 8 == MP_MLP_PACKED_BYTES, "packed size");
@@ -484,10 +488,10 @@ __global__ __launch_bounds__(128) void dataset_kernel(
     t[5] = (float)((m[3] - mR0) / dt);
 }
 
-// A cell that failed the check: its state after the first plain substep (FP64, so the
-// remaining substeps continue exactly as the plain loop would), its absorbed power and
-// its index.  Fallback cells are compacted into a list and finished by a second kernel,
-// so the substep loop runs on full waves instead of diverging inside the first.
+// A cell that failed the check: its start state (FP64, as the plain loop holds it), its
+// absorbed power and its index.  Fallback cells are compacted into a list and take every
+// plain substep in a second kernel, so the substep loop runs on full waves instead of
+// diverging inside the first.
 struct FallbackRec {
     double s[11];  // te, tl, tsT, tsR, m[6], pabs
     int64_t cell;
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(128) void surrogate_fallback_kernel(
         double m[6] = {r.s[4], r.s[5], r.s[6], r.s[7], r.s[8], r.s[9]};
         const double pabs = r.s[10];
         const int64_t n = r.cell;
-        for (int k = 1; k < subcycles; ++k)
+        for (int k = 0; k < subcycles; ++k)
             plain_substep<L>(te, tl, tsT, tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
         Te[n] = (L)te; Tl[n] = (L)tl; TsT[n] = (L)tsT; TsR[n] = (L)tsR;
         mTx[n] = (L)m[0]; mTy[n] = (L)m[1]; mTz[n] = (L)m[2];
@@ -534,7 +538,10 @@ __global__ __launch_bounds__(256) void surrogate_kernel(
     __shared__ float ys[4][64][8];
     load_infer_lds(s, packed, P);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // the blocks loop over the cells 256 at a time (a lane past the end only occurs in a
+    // block's last pass, so its early return skips no later barrier)
+    for (int64_t base = (int64_t)blockIdx.x * 256; base < N; base += (int64_t)gridDim.x * 256) {
+    const int64_t n = base + threadIdx.x;
     const bool active = n < N;
 
     double te = 300.0, tl = 300.0, tsT = 300.0, tsR = 300.0, pabs = 0.0;
@@ -558,45 +565,46 @@ __global__ __launch_bounds__(256) void surrogate_kernel(
     wave_infer(s, xs[w], ys[w], norm, r);  // contains the block's barriers
     if (!active) return;
 
-    const double te0 = te, tl0 = tl, tsT0 = tsT, tsR0 = tsR;
-    const double m0[6] = {m[0], m[1], m[2], m[3], m[4], m[5]};
     const double sub_dt = dt / (double)subcycles;
-    plain_substep<L>(te, tl, tsT, tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
 
-    // Trapezoid check of the network's whole step: with r0 the plain one-substep rate at
-    // the start and r1 the plain one-substep rate at the network's end state, the step
-    // must match dt/2 (r0 + r1) to tol (absolute in m, relative in T).
+    // Check of the network's whole step against one plain substep taken at its midpoint (the
+    // midpoint rule, second order like the step it checks): temperatures and m_x of the
+    // midpoint state lie halfway along the predicted step, the transverse m at their start
+    // value.  With r_mid the plain one-substep rate there, the predicted change must match
+    // dt r_mid to tol (absolute in m, relative in T).  The transverse m, which the network
+    // does not predict, advance by dt r_mid.
     bool accept = tol >= 0.0;
     const double hg = 0.5 * gd.gamma_e;
-    const double ue0 = hg * (te0 * te0);
+    const double ue0 = hg * (te * te);
     const double S = (double)subcycles;
     double e_te = 0, e_tl = 0, e_tsT = 0, e_tsR = 0, e_m[6] = {0, 0, 0, 0, 0, 0};
     if (accept) {
         e_te = sqrt(jl_max(hg, ue0 + dt * (pabs + (double)r[0])) / hg);
-        e_tl = jl_max(1.0, tl0 + dt * (double)r[1]);
-        e_tsT = jl_max(1.0, tsT0 + dt * (double)r[2]);
-        e_tsR = jl_max(1.0, tsR0 + dt * (double)r[3]);
-        e_m[0] = m0[0] + dt * (double)r[4];
-        e_m[3] = m0[3] + dt * (double)r[5];
-        e_m[1] = m0[1] + S * (m[1] - m0[1]);
-        e_m[2] = m0[2] + S * (m[2] - m0[2]);
-        e_m[4] = m0[4] + S * (m[4] - m0[4]);
-        e_m[5] = m0[5] + S * (m[5] - m0[5]);
+        e_tl = jl_max(1.0, tl + dt * (double)r[1]);
+        e_tsT = jl_max(1.0, tsT + dt * (double)r[2]);
+        e_tsR = jl_max(1.0, tsR + dt * (double)r[3]);
+        e_m[0] = m[0] + dt * (double)r[4];
+        e_m[3] = m[3] + dt * (double)r[5];
         accept = __builtin_isfinite(e_te) && __builtin_isfinite(e_tl) && __builtin_isfinite(e_tsT) &&
                  __builtin_isfinite(e_tsR) && __builtin_fabs(e_m[0]) <= 1.0 && __builtin_fabs(e_m[3]) <= 1.0;
     }
     if (accept) {
-        double b_te = e_te, b_tl = e_tl, b_tsT = e_tsT, b_tsR = e_tsR;
-        double b_m[6] = {e_m[0], e_m[1], e_m[2], e_m[3], e_m[4], e_m[5]};
+        const double h_te = 0.5 * (te + e_te), h_tl = 0.5 * (tl + e_tl), h_tsT = 0.5 * (tsT + e_tsT),
+                     h_tsR = 0.5 * (tsR + e_tsR);
+        const double h_m[6] = {0.5 * (m[0] + e_m[0]), m[1], m[2], 0.5 * (m[3] + e_m[3]), m[4], m[5]};
+        double b_te = h_te, b_tl = h_tl, b_tsT = h_tsT, b_tsR = h_tsR;
+        double b_m[6] = {h_m[0], h_m[1], h_m[2], h_m[3], h_m[4], h_m[5]};
         plain_substep<L>(b_te, b_tl, b_tsT, b_tsR, b_m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, lut_N, sub_dt, iters);
-        const double hS = 0.5 * S;
-        double res = __builtin_fabs(e_te - te0 - hS * ((te - te0) + (b_te - e_te))) / e_te;
-        res = jl_max(res, __builtin_fabs(e_tl - tl0 - hS * ((tl - tl0) + (b_tl - e_tl))) / e_tl);
-        res = jl_max(res, __builtin_fabs(e_tsT - tsT0 - hS * ((tsT - tsT0) + (b_tsT - e_tsT))) / e_tsT);
-        res = jl_max(res, __builtin_fabs(e_tsR - tsR0 - hS * ((tsR - tsR0) + (b_tsR - e_tsR))) / e_tsR);
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            res = jl_max(res, __builtin_fabs(e_m[c] - m0[c] - hS * ((m[c] - m0[c]) + (b_m[c] - e_m[c]))));
+        double res = __builtin_fabs(e_te - te - S * (b_te - h_te)) / e_te;
+        res = jl_max(res, __builtin_fabs(e_tl - tl - S * (b_tl - h_tl)) / e_tl);
+        res = jl_max(res, __builtin_fabs(e_tsT - tsT - S * (b_tsT - h_tsT)) / e_tsT);
+        res = jl_max(res, __builtin_fabs(e_tsR - tsR - S * (b_tsR - h_tsR)) / e_tsR);
+        res = jl_max(res, __builtin_fabs(e_m[0] - m[0] - S * (b_m[0] - h_m[0])));
+        res = jl_max(res, __builtin_fabs(e_m[3] - m[3] - S * (b_m[3] - h_m[3])));
+        e_m[1] = m[1] + S * (b_m[1] - h_m[1]);
+        e_m[2] = m[2] + S * (b_m[2] - h_m[2]);
+        e_m[4] = m[4] + S * (b_m[4] - h_m[4]);
+        e_m[5] = m[5] + S * (b_m[5] - h_m[5]);
         accept = res <= tol;  // false for NaN
     }
     const unsigned long long mask = __ballot(!accept);
@@ -627,6 +635,7 @@ __global__ __launch_bounds__(256) void surrogate_kernel(
         Te[n] = (L)e_te; Tl[n] = (L)e_tl; TsT[n] = (L)e_tsT; TsR[n] = (L)e_tsR;
         mTx[n] = (L)e_m[0]; mTy[n] = (L)e_m[1]; mTz[n] = (L)e_m[2];
         mRx[n] = (L)e_m[3]; mRy[n] = (L)e_m[4]; mRz[n] = (L)e_m[5];
+    }
     }
 }
 
@@ -689,15 +698,20 @@ static int launch_surrogate(mp_film *f, const mp_ferri_params *p, const mp_lut *
                 (L *)f->m_TM_z, (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, f->window_sum, gc,    \
                 (const L *)lut->tm, (const L *)lut->re, tmin, invdt, lut->n, dtc, subcycles, iters, N, \
                 (const uint2 *)m->packed, m->params, m->norm, tol, (unsigned long long *)fallbacks, g_recs, g_nrec
-    if (has_pabs) surrogate_kernel<L, true><<<cdiv(N, 256), 256, 0, stream()>>>(SG_ARGS);
-    else surrogate_kernel<L, false><<<cdiv(N, 256), 256, 0, stream()>>>(SG_ARGS);
+    // at most SUR_BLOCKS blocks resident for the whole update, looping over the cells: the
+    // update then holds a fixed share of each SIMD beside the Yee kernels of the next step
+    // instead of filling every CU (10 240 blocks at the 2.62 M-cell film)
+    const unsigned grid = (unsigned)std::min<int64_t>(cdiv(N, 256), SUR_BLOCKS);
+    if (has_pabs) surrogate_kernel<L, true><<<grid, 256, 0, stream()>>>(SG_ARGS);
+    else surrogate_kernel<L, false><<<grid, 256, 0, stream()>>>(SG_ARGS);
 #undef SG_ARGS
     MP_LAUNCH_CHECK("surrogate_kernel");
     {
-        // the cells that failed the check finish their remaining substeps (none when
-        // subcycles == 1: the first plain substep is then the whole plain step)
+        // the cells that failed the check take the plain integrator's substeps from their
+        // start state; the grid loops over the list, so a step with few fallbacks costs a
+        // small launch (it waits for room beside the next step's Yee kernels)
         KernelTimer t2("surrogate_fallback");
-        const int64_t blocks = std::min<int64_t>(cdiv(N, 128), 4096);
+        const int64_t blocks = std::min<int64_t>(cdiv(N, 128), 512);
         surrogate_fallback_kernel<L><<<blocks, 128, 0, stream()>>>(
             (L *)f->Te, (L *)f->Tl, (L *)f->Ts_TM, (L *)f->Ts_RE, (L *)f->m_TM_x, (L *)f->m_TM_y, (L *)f->m_TM_z,
             (L *)f->m_RE_x, (L *)f->m_RE_y, (L *)f->m_RE_z, g_recs, g_nrec, gc, (const L *)lut->tm,

@@ -85,52 +85,49 @@ static int64_t orc_surrogate_##LT(mp_film *f, const mp_ferri_params *gd, const L
         double m[6] = {(double)mTx[n], (double)mTy[n], (double)mTz[n],                        \
                        (double)mRx[n], (double)mRy[n], (double)mRz[n]};                       \
         const float *r = rates + n * 6;                                                       \
-        const double te0 = te, tl0 = tl, tsT0 = tsT, tsR0 = tsR;                              \
-        double m0[6];                                                                         \
-        for (int c = 0; c < 6; ++c) m0[c] = m[c];                                             \
         const double sub_dt = dt / (double)subcycles;                                         \
-        orc_plain_substep_##LT(&te, &tl, &tsT, &tsR, m, pabs, gd, tm_lut, re_lut, T_min, inv_dT, \
-                               lut_N, sub_dt, iters);                                         \
-        /* trapezoid check of the network's step against one plain substep at each end */     \
+        /* midpoint check of the network's step against one plain substep */                  \
         int accept = tol >= 0.0;                                                              \
         const double hg = 0.5 * gd->gamma_e;                                                  \
-        const double ue0 = hg * (te0 * te0);                                                  \
+        const double ue0 = hg * (te * te);                                                    \
         const double S = (double)subcycles;                                                   \
         double e_te = 0, e_tl = 0, e_tsT = 0, e_tsR = 0, e_m[6] = {0, 0, 0, 0, 0, 0};         \
         if (accept) {                                                                         \
             e_te = sqrt(jl_max(hg, ue0 + dt * (pabs + (double)r[0])) / hg);                   \
-            e_tl = jl_max(1.0, tl0 + dt * (double)r[1]);                                      \
-            e_tsT = jl_max(1.0, tsT0 + dt * (double)r[2]);                                    \
-            e_tsR = jl_max(1.0, tsR0 + dt * (double)r[3]);                                    \
-            e_m[0] = m0[0] + dt * (double)r[4];                                               \
-            e_m[3] = m0[3] + dt * (double)r[5];                                               \
-            e_m[1] = m0[1] + S * (m[1] - m0[1]);                                              \
-            e_m[2] = m0[2] + S * (m[2] - m0[2]);                                              \
-            e_m[4] = m0[4] + S * (m[4] - m0[4]);                                              \
-            e_m[5] = m0[5] + S * (m[5] - m0[5]);                                              \
+            e_tl = jl_max(1.0, tl + dt * (double)r[1]);                                       \
+            e_tsT = jl_max(1.0, tsT + dt * (double)r[2]);                                     \
+            e_tsR = jl_max(1.0, tsR + dt * (double)r[3]);                                     \
+            e_m[0] = m[0] + dt * (double)r[4];                                                \
+            e_m[3] = m[3] + dt * (double)r[5];                                                \
             accept = isfinite(e_te) && isfinite(e_tl) && isfinite(e_tsT) && isfinite(e_tsR) &&\
                      fabs(e_m[0]) <= 1.0 && fabs(e_m[3]) <= 1.0;                              \
         }                                                                                     \
         if (accept) {                                                                         \
-            double b_te = e_te, b_tl = e_tl, b_tsT = e_tsT, b_tsR = e_tsR;                    \
+            const double h_te = 0.5 * (te + e_te), h_tl = 0.5 * (tl + e_tl);                  \
+            const double h_tsT = 0.5 * (tsT + e_tsT), h_tsR = 0.5 * (tsR + e_tsR);            \
+            const double h_m[6] = {0.5 * (m[0] + e_m[0]), m[1], m[2], 0.5 * (m[3] + e_m[3]), m[4], m[5]};\
+            double b_te = h_te, b_tl = h_tl, b_tsT = h_tsT, b_tsR = h_tsR;                    \
             double b_m[6];                                                                    \
-            for (int c = 0; c < 6; ++c) b_m[c] = e_m[c];                                      \
+            for (int c = 0; c < 6; ++c) b_m[c] = h_m[c];                                      \
             orc_plain_substep_##LT(&b_te, &b_tl, &b_tsT, &b_tsR, b_m, pabs, gd, tm_lut, re_lut, T_min,\
                                    inv_dT, lut_N, sub_dt, iters);                             \
-            const double hS = 0.5 * S;                                                        \
-            double res = fabs(e_te - te0 - hS * ((te - te0) + (b_te - e_te))) / e_te;         \
-            res = jl_max(res, fabs(e_tl - tl0 - hS * ((tl - tl0) + (b_tl - e_tl))) / e_tl);   \
-            res = jl_max(res, fabs(e_tsT - tsT0 - hS * ((tsT - tsT0) + (b_tsT - e_tsT))) / e_tsT);\
-            res = jl_max(res, fabs(e_tsR - tsR0 - hS * ((tsR - tsR0) + (b_tsR - e_tsR))) / e_tsR);\
-            for (int c = 0; c < 6; ++c)                                                       \
-                res = jl_max(res, fabs(e_m[c] - m0[c] - hS * ((m[c] - m0[c]) + (b_m[c] - e_m[c]))));\
+            double res = fabs(e_te - te - S * (b_te - h_te)) / e_te;                          \
+            res = jl_max(res, fabs(e_tl - tl - S * (b_tl - h_tl)) / e_tl);                    \
+            res = jl_max(res, fabs(e_tsT - tsT - S * (b_tsT - h_tsT)) / e_tsT);               \
+            res = jl_max(res, fabs(e_tsR - tsR - S * (b_tsR - h_tsR)) / e_tsR);               \
+            res = jl_max(res, fabs(e_m[0] - m[0] - S * (b_m[0] - h_m[0])));                   \
+            res = jl_max(res, fabs(e_m[3] - m[3] - S * (b_m[3] - h_m[3])));                   \
+            e_m[1] = m[1] + S * (b_m[1] - h_m[1]);                                            \
+            e_m[2] = m[2] + S * (b_m[2] - h_m[2]);                                            \
+            e_m[4] = m[4] + S * (b_m[4] - h_m[4]);                                            \
+            e_m[5] = m[5] + S * (b_m[5] - h_m[5]);                                            \
             accept = res <= tol;                                                              \
         }                                                                                     \
         if (accept) {                                                                         \
             te = e_te; tl = e_tl; tsT = e_tsT; tsR = e_tsR;                                   \
             for (int c = 0; c < 6; ++c) m[c] = e_m[c];                                        \
         } else {                                                                              \
-            for (int k = 1; k < subcycles; ++k)                                               \
+            for (int k = 0; k < subcycles; ++k)                                               \
                 orc_plain_substep_##LT(&te, &tl, &tsT, &tsR, m, pabs, gd, tm_lut, re_lut, T_min,\
                                        inv_dT, lut_N, sub_dt, iters);                         \
             fallbacks += 1;                                                                   \

@@ -119,7 +119,7 @@ def test_coupled_pump_bf16_storage_full_size():
 def test_surrogate_step_full_film():
     """The learned film update on the 2.62 M-cell film of the 512 x 512 x 256 pump:
     mp_surrogate_step on every cell against orc_surrogate_apply on every 61st cell, from
-    seeded film states, with all steps accepted (tol 1e9) and with the trapezoid check at
+    seeded film states, with all steps accepted (tol 1e9) and with the midpoint check at
     1e-4 (a mix of accepted and fallback cells)."""
     import ctypes as C
     import driver

@@ -310,15 +310,18 @@ def test_coupled_pump_batched_run_overlap():
     gpu.free()
 
 
-def test_batched_dft_record_is_bitwise_per_step():
+@pytest.mark.parametrize("shape", [(40, 36, 64), (44, 40, 78)])
+def test_batched_dft_record_is_bitwise_per_step(shape):
     """Inside one mp_run_steps call the DFT record is batched (snapshots, then one pass over
     the stored steps in step order); one step per call records step by step.  The two must
-    give the same bits."""
+    give the same bits.  The y-normal planes are snapshot in 32 x 32 tiles: the second shape
+    leaves partial tiles along both of their axes."""
     from mphip import configs
     from mphip.monitors import DFTPlanes
     from mphip.state import FDTDState
-    spec, meta = configs.coupled_pump(nx=40, ny=36, nz=64, steps=200, fluence_mj_cm2=12.0, npml=6, src_z=40e-9)
-    print("case batched DFT record vs per step 40x36x64", flush=True)
+    nx, ny, nz = shape
+    spec, meta = configs.coupled_pump(nx=nx, ny=ny, nz=nz, steps=200, fluence_mj_cm2=12.0, npml=6, src_z=40e-9)
+    print(f"case batched DFT record vs per step {nx}x{ny}x{nz}", flush=True)
     a, b = FDTDState(spec), FDTDState(spec)
     ma = DFTPlanes(a, meta["planes"], meta["omega_bins"])
     mb = DFTPlanes(b, meta["planes"], meta["omega_bins"])

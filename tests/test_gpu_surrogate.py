@@ -197,10 +197,12 @@ def test_trained_surrogate_gain_on_relax():
     128 x 128 x 64 accuracy case's pump at 35, 50, 70 and 85 mJ/cm^2, every 5th step, every
     4th film cell (about 2e7 rows).  The network is trained on the GPU (30 000 Adam steps) and
     evaluated on the held-out 60 mJ/cm^2 run against the FP64 oracle's film-mean
-    magnetisation.  The surrogate step costs the network plus 2 plain substeps per accepted
-    cell; without the network, 1 or 2 plain substeps per 5 fs step switch the wrong cells.
-    Plain with 3 substeps is printed beside it: at this step size it is both cheaper and
-    more accurate than the surrogate (measured, not asserted).
+    magnetisation.  The surrogate step costs the network plus 1 plain substep per accepted
+    cell (the midpoint check); without the network, 1 or 2 plain substeps per 5 fs step
+    switch the wrong cells.  At the check tolerance of the workload table (1e-4) the
+    surrogate meets the 1e-3 bar.  Measured, not asserted: at 1e-3 the check lets steps
+    through in the switching cells (film-mean |dm| 5e-2), and plain with 3 substeps is both
+    cheaper and more accurate than the surrogate at this step size.
     """
     import driver
     from mphip import configs
@@ -219,11 +221,11 @@ def test_trained_surrogate_gain_on_relax():
     for k, v in res.items():
         print(k, v, flush=True)
     assert res["plain_10sub"]["film_mean_dm"] < 1e-4   # FP32 against FP64
-    e_sur = res["surrogate_tol0.001"]["film_mean_dm"]
+    e_sur = res["surrogate_tol0.0001"]["film_mean_dm"]
+    assert e_sur < 1e-3
     assert e_sur < res["plain_1sub"]["film_mean_dm"] / 100
-    assert e_sur < res["plain_2sub"]["film_mean_dm"] / 5
-    assert res["surrogate_tol0.0001"]["film_mean_dm"] < 5e-3
-    assert res["surrogate_tol0.001"]["fallback"] < 0.5
+    assert e_sur < res["plain_2sub"]["film_mean_dm"] / 100
+    assert res["surrogate_tol0.0001"]["fallback"] < 0.8
     sur.free()
 
 
