@@ -314,8 +314,7 @@ def test_coupled_pump_batched_run_overlap():
 def test_batched_dft_record_is_bitwise_per_step(shape):
     """Inside one mp_run_steps call the DFT record is batched (snapshots, then one pass over
     the stored steps in step order); one step per call records step by step.  The two must
-    give the same bits.  The y-normal planes are snapshot in 32 x 32 tiles: the second shape
-    leaves partial tiles along both of their axes."""
+    give the same bits.  The second shape has no axis a multiple of 32 cells."""
     from mphip import configs
     from mphip.monitors import DFTPlanes
     from mphip.state import FDTDState
