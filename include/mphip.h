@@ -580,6 +580,12 @@ int mp_sum_blocks(const double *x, int64_t block, int64_t nblocks, double *out);
  * z -> field[:,:,i] as (nx, ny), first index fastest. */
 int mp_plane_gather(const void *field, int32_t dtype, int64_t nx, int64_t ny, int64_t nz,
                     int32_t axis, int64_t index, void *out);
+/* |field| of one plane as float (device), in the plane order of mp_plane_gather: the movie
+ * frame staging of the HDF5 frame writers, `dev .= abs.(view(field, ...))` followed by the
+ * Float32 conversion (_stage_abs_frame!, ext/MagnetoPhotonicHDF5Ext.jl:619-624, in
+ * record_production_frame! :626-636 and record_probe_frame! :685-697). */
+int mp_plane_abs(const void *field, int32_t dtype, int64_t nx, int64_t ny, int64_t nz,
+                 int32_t axis, int64_t index, float *out);
 
 /* probe_trace_plane_kernel! (src/fdtd/Kernels.jl:467-478): work (ny, nz, 4), device FP64,
  * = Ey[i_pre]*w, Ez[i_pre]*w, Ey[i_post]*w, Ez[i_post]*w with w = mode_w*area (ny*nz

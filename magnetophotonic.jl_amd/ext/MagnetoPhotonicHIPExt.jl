@@ -79,6 +79,15 @@ function Base.Array(a::HIPArray{T,N}) where {T,N}
     return out
 end
 
+function Base.copyto!(dst::Array{T}, src::HIPArray{T}) where {T}
+    length(dst) == length(src) || throw(DimensionMismatch("copyto! sizes differ"))
+    isempty(src) || _check(ccall((:mp_memcpy_d2h, libmphip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t),
+                                 dst, src.ptr, sizeof(dst)), "mp_memcpy_d2h")
+    return dst
+end
+
+Base.similar(::HIPArray, ::Type{S}, dims::Dims{N}) where {S,N} = HIPArray{S,N}(dims)
+
 function Base.fill!(a::HIPArray{T}, v) where {T<:Union{Float32,Float64}}
     _check(ccall((:mp_fill, libmphip), Cint, (Ptr{Cvoid}, Int32, Int64, Cdouble),
                  a.ptr, _dtype(T), length(a), Float64(v)), "mp_fill")
@@ -121,8 +130,35 @@ function Base.copyto!(dest::HIPArray{T}, bc::Broadcast.Broadcasted{HIPStyle}) wh
         (src isa HIPArray{T} || src isa Array{T}) && return copyto!(dest, src)
         src isa Number && return fill!(dest, src)
     end
+    if bc.f === abs && length(bc.args) == 1 && dest isa HIPArray{Float32,2}
+        ax = _plane_view(bc.args[1])
+        if ax !== nothing
+            f, a, i = ax
+            nx, ny, nz = size(f)
+            _check(ccall((:mp_plane_abs, libmphip), Cint,
+                         (Ptr{Cvoid}, Int32, Int64, Int64, Int64, Int32, Int64, Ptr{Cvoid}),
+                         _p(f), _dtype(eltype(f)), nx, ny, nz, Int32(a), Int64(i) - 1, _p(dest)), "mp_plane_abs")
+            return dest
+        end
+    end
     throw(ArgumentError("HIPArray broadcasting supports dest .= src (a HIPArray or Array of the same " *
-                        "element type) and dest .= scalar"))
+                        "element type), dest .= scalar and dest .= abs.(view(field, plane)) into Float32"))
+end
+
+# (field, axis 0/1/2, 1-based index) of a whole-plane view of a 3-D HIPArray
+# (view(f, i, :, :), view(f, :, j, :), view(f, :, :, k)): the planes whose |values| the HDF5
+# frame writers stage with dev .= abs.(view) (ext/MagnetoPhotonicHDF5Ext.jl:619-624).
+function _plane_view(v)
+    v isa SubArray || return nothing
+    f = parent(v)
+    (f isa HIPArray && ndims(f) == 3) || return nothing
+    I = parentindices(v)
+    whole(i, n) = i isa Base.Slice || (i isa AbstractUnitRange && first(i) == 1 && length(i) == n)
+    nx, ny, nz = size(f)
+    I[1] isa Integer && whole(I[2], ny) && whole(I[3], nz) && return (f, 0, Int(I[1]))
+    whole(I[1], nx) && I[2] isa Integer && whole(I[3], nz) && return (f, 1, Int(I[2]))
+    whole(I[1], nx) && whole(I[2], ny) && I[3] isa Integer && return (f, 2, Int(I[3]))
+    return nothing
 end
 
 # ---- backend hooks (src/core/Backend.jl) --------------------------------------------------------

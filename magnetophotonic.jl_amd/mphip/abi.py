@@ -128,6 +128,7 @@ HOOK_SIGNATURES = {
     "reduce": (C.c_int, [vp, vp, vp, i32, i64, i32, vp]),
     "sum_blocks": (C.c_int, [vp, i64, i64, vp]),
     "plane_gather": (C.c_int, [vp, i32, i64, i64, i64, i32, i64, vp]),
+    "plane_abs": (C.c_int, [vp, i32, i64, i64, i64, i32, i64, vp]),
     "probe_trace_samples": (C.c_int, [P(Yee), i64, i64, vp, vp, vp]),
     "probe_frames": (C.c_int, [P(Yee), i64, i64, i64, vp, vp, vp, vp, vp, vp]),
 }

@@ -25,6 +25,11 @@ class FDTDParams:
     c0: float = C0
     mu0: float = MU0
     eps0: float = EPS0
+    n_si3n4: float = 2.0
+    epsr_si3n4: float = 4.0
+    n_sio2: float = 1.444
+    epsr_sio2: float = 1.444 * 1.444
+    epsr_vac: float = 1.0
 
 
 def get_n_sio2(lambda_m: float) -> float:

@@ -87,6 +87,8 @@ class StateSpec:
     absorption_model: str
     has_film: bool
     extras: dict = field(default_factory=dict)
+    epsr: Optional[np.ndarray] = None   # (nx, ny, nz) CT relative permittivity, the state's epsr (Solver.jl:99)
+    film_fill: Optional[np.ndarray] = None   # CT fill fraction of each film cell (region.material_fill, Solver.jl:148)
 
     @property
     def shape(self):
@@ -208,7 +210,9 @@ def build_state_spec(grid: Grid3D, geo: Geometry, dt: float, params: FDTDParams 
                      pos=pos, film_init=film_init, gd=gd, lut=lut,
                      multiphysics_every=int(multiphysics_every), subcycles=int(subs),
                      brillouin_iters=int(brillouin_iters), multiphysics_enabled=bool(enable_multiphysics),
-                     absorption_model=absorption_model, has_film=has_film)
+                     absorption_model=absorption_model, has_film=has_film,
+                     epsr=np.asarray(geo.epsr, order="F").astype(CT, order="F"),
+                     film_fill=(lst.fill.copy() if has_film else np.zeros(0, CT)))
 
 
 def _small_codes(a: np.ndarray, limit: int):
