@@ -102,10 +102,14 @@ static int film_update(mp_state *s, double dt_mp) {
 }
 
 // About 10 us on one wave of the main stream (3 x s_sleep 127, 64 clocks each at ~2.4 GHz).
-// Issued right after the film update when no DFT snapshot follows: the film update's
-// workgroups then reach the CUs before the next H update's.  When both are dispatched
-// together the 512 x 512 x 256 step measured 1.74 ms, with this head start 1.51 ms (the
-// 55 us snapshot gives the same head start); an empty launch in its place changed nothing.
+// Issued right after the film update: the film update's workgroups then reach the CUs
+// before those of the next launches on the main stream (the DFT snapshot, the H update).
+// Without a DFT the 512 x 512 x 256 step measured 1.74 ms with both dispatched together,
+// 1.51 ms with this head start.  With the DFT the snapshot alone gave the head start in
+// some builds (1.487 ms) and not in others that differed only in the monitor code object's
+// layout (1.52-1.56 ms); with this launch every such build measured 1.489 ms.  Shorter
+// sleeps (one s_sleep, or an empty launch) measured 1.60 ms, longer ones (4-8) 1-15 us more
+// per step than two or three.
 __global__ void head_start_kernel() {
     for (int r = 0; r < 3; ++r) __builtin_amdgcn_s_sleep(127);
 }
@@ -121,10 +125,8 @@ static int launch_film_side(mp_state *s, double dt_mp) {
     if (rc) return rc;
     MP_HIP(hipEventRecord(g_ov.film_done, g_ov.side));
     g_ov.pending = true;
-    if (!(s->record_dft && s->nplanes > 0)) {
-        head_start_kernel<<<1, 64, 0, stream()>>>();
-        MP_LAUNCH_CHECK("head_start_kernel");
-    }
+    head_start_kernel<<<1, 64, 0, stream()>>>();
+    MP_LAUNCH_CHECK("head_start_kernel");
     return MP_OK;
 }
 
