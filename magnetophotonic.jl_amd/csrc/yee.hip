@@ -721,6 +721,24 @@ __device__ __forceinline__ void box_pos(uint32_t t, uint32_t per_plane, uint32_t
     gg = r - jj * ngi;
 }
 
+// Loads and stores at a 32-bit byte offset from a uniform base: every volume of the interior
+// and fused kernels is then addressed by one offset register and its base in scalar
+// registers (the interior E kernel in 63 VGPRs instead of 66: three of its waves fit each
+// SIMD beside the film update instead of two; the interior H kernel, at 64 either way, keeps
+// 64-bit element offsets, measured 2% faster alone).  The launchers take these kernels only
+// for volumes below 4 GiB.
+template <typename T>
+__device__ __forceinline__ const T &at(const float *base, uint32_t boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+__device__ __forceinline__ float4 ldb4(const float *base, uint32_t boff) { return at<float4>(base, boff); }
+__device__ __forceinline__ float4 ldb4nt(const float *base, uint32_t boff) {
+    return ld4nt(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff));
+}
+__device__ __forceinline__ void stb4nt(float *base, uint32_t boff, float4 v) {
+    st4nt(reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff), v);
+}
+
 __global__ __launch_bounds__(256) void yee_H_int_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
@@ -792,31 +810,31 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
     const bool act = t < ntot;
     uint32_t kk, jj, gg;
     box_pos(act ? t : 0u, per_plane, ngi, kk, jj, gg);
-    const int64_t i0 = (int64_t)(gi0 + gg) * VX, j = (int64_t)(j0 + jj), k = (int64_t)(k0 + kk);
-    const int64_t sy = nx, sz = nx * ny;
-    const int64_t o = i0 + sy * j + sz * k;
-    const float4 hx_p = ld4(Hx + o - sz), hy_p = ld4(Hy + o - sz);
-    const float4 hx_c = ld4(Hx + o), hy_c = ld4(Hy + o), hz_c = ld4(Hz + o);
-    const float4 hz_y = ld4(Hz + o - sy), hx_y = ld4(Hx + o - sy);
-    float4 dx = ld4nt(Dx + o), dy = ld4nt(Dy + o), dz = ld4nt(Dz + o);
+    const uint32_t i0 = (gi0 + gg) * VX, j = j0 + jj, k = k0 + kk;
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)(nx * ny);
+    const uint32_t ob = (i0 + sy * j + sz * k) * 4u;
+    const float4 hx_p = ldb4(Hx - sz, ob), hy_p = ldb4(Hy - sz, ob);
+    const float4 hx_c = ldb4(Hx, ob), hy_c = ldb4(Hy, ob), hz_c = ldb4(Hz, ob);
+    const float4 hz_y = ldb4(Hz - sy, ob), hx_y = ldb4(Hx - sy, ob);
+    float4 dx = ldb4nt(Dx, ob), dy = ldb4nt(Dy, ob), dz = ldb4nt(Dz, ob);
     // the previous lane holds the previous group of this row unless this is its first group
     float hz_px = __shfl_up(hz_c.w, 1), hy_px = __shfl_up(hy_c.w, 1);
     if ((threadIdx.x & 63) == 0 || gg == 0) {
-        hz_px = Hz[o - 1];
-        hy_px = Hy[o - 1];
+        hz_px = at<float>(Hz - 1, ob);
+        hy_px = at<float>(Hy - 1, ob);
     }
-    const float4 idx4 = ld4(iddx + i0), xk4 = ld4(xk + i0);
+    const float4 idx4 = ldb4(iddx, i0 * 4u), xk4 = ldb4(xk, i0 * 4u);
     const float idy_ = iddy[j], yk_ = yk[j], idz_ = iddz[k], zk_ = zk[k];
     // inv_eps (or the material bytes) load with the fields; the table lookup follows the cell
     // loop, so no load waits before the last one is issued
     float4 iex4, iey4, iez4;
     uint32_t m4 = 0u;
     if (MI) {
-        m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + o));
+        m4 = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(mat + (ob >> 2)));
     } else {
-        iex4 = ld4nt(iex + o);
-        iey4 = ld4nt(iey + o);
-        iez4 = ld4nt(iez + o);
+        iex4 = ldb4nt(iex, ob);
+        iey4 = ldb4nt(iey, ob);
+        iez4 = ldb4nt(iez, ob);
     }
 #pragma unroll
     for (int e = 0; e < VX; ++e) {
@@ -845,10 +863,10 @@ __global__ __launch_bounds__(256) void yee_E_int_kernel(
         iez4 = make_float4(t0.z, t1.z, t2.z, t3.z);
     }
     if (act) {
-        st4nt(Dx + o, dx); st4nt(Dy + o, dy); st4nt(Dz + o, dz);
-        st4nt(Ex + o, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
-        st4nt(Ey + o, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
-        st4nt(Ez + o, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
+        stb4nt(Dx, ob, dx); stb4nt(Dy, ob, dy); stb4nt(Dz, ob, dz);
+        stb4nt(Ex, ob, make_float4(dx.x * iex4.x, dx.y * iex4.y, dx.z * iex4.z, dx.w * iex4.w));
+        stb4nt(Ey, ob, make_float4(dy.x * iey4.x, dy.y * iey4.y, dy.z * iey4.z, dy.w * iey4.w));
+        stb4nt(Ez, ob, make_float4(dz.x * iez4.x, dz.y * iez4.y, dz.z * iez4.z, dz.w * iez4.w));
     }
 }
 
@@ -875,20 +893,6 @@ struct PlanePtrs {
 // E once: 73 bytes instead of the 85 of the two interior launches.  The per-cell arithmetic is
 // that of yee_H_int_kernel and yee_E_int_kernel, operation for operation.
 constexpr int FL = 512;   // lanes per workgroup (8 waves: two fit each SIMD beside the film update)
-
-// Loads and stores at a 32-bit byte offset from a uniform base: every volume of the fused
-// kernel is then addressed by one offset register and its base in scalar registers.
-template <typename T>
-__device__ __forceinline__ const T &at(const float *base, uint32_t boff) {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
-}
-__device__ __forceinline__ float4 ldb4(const float *base, uint32_t boff) { return at<float4>(base, boff); }
-__device__ __forceinline__ float4 ldb4nt(const float *base, uint32_t boff) {
-    return ld4nt(reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + boff));
-}
-__device__ __forceinline__ void stb4nt(float *base, uint32_t boff, float4 v) {
-    st4nt(reinterpret_cast<float *>(reinterpret_cast<char *>(base) + boff), v);
-}
 
 template <bool MI>
 __global__ __launch_bounds__(FL) void yee_HE_int_kernel(
@@ -1145,8 +1149,10 @@ static Box interior_box(const mp_yee *g, const mp_cpml *c) {
 // The H and E updates split into interior and shell launches when the box is non-empty and
 // every group index, lanes past the end included, fits 32 bits; kernel_form
 // MP_YEE_WHOLE keeps one launch over the whole volume.
+// The interior kernels address the volumes by 32-bit byte offsets: volumes of 4 GiB or more
+// (2^30 cells) take the general kernels over the whole volume.
 static bool split_ok(const mp_yee *g, const Box &b) {
-    return g->kernel_form != MP_YEE_WHOLE && b.n() > 0 && (g->nx / VX) * g->ny * g->nz < ((int64_t)1 << 32) - 4096;
+    return g->kernel_form != MP_YEE_WHOLE && b.n() > 0 && 4 * g->nx * g->ny * g->nz < ((int64_t)1 << 32);
 }
 
 // Shell layout of a box (see ShellGeom); n = 0 when a divisor would be below 2, where the
