@@ -140,7 +140,10 @@ static int film_update_due(mp_state *s);
 static int one_step(mp_state *s, const mp_yee *in, const mp_yee *out) {
     int rc;
     if (in == out) {
-        if ((rc = mp_update_H_3d(in, s->cpml, s->dt, s->mu0))) return rc;
+        yee_shell_split_x(g_ov.pending);   // the film update of the previous step runs beside
+        rc = mp_update_H_3d(in, s->cpml, s->dt, s->mu0);
+        yee_shell_split_x(false);
+        if (rc) return rc;
         if ((rc = inject(s, in, in))) return rc;
         if ((rc = mp_update_E_3d(in, s->cpml, s->dt))) return rc;
     } else {

@@ -20,4 +20,9 @@ int yee_fused_E_shell(const mp_yee *out, mp_cpml *c, double dt);
 // e_only) from one set to the other.  H there changes only under an H source, E also under
 // the source and the film list work, so a ping-pong step first carries E there over.
 int yee_sync_boundary(const mp_yee *from, const mp_yee *to, bool e_only);
+// Whether the next in-place H updates split the rest of their CPML shell by x (the box's x
+// range without x-slab terms in a leaner kernel, then the x edges).  The step driver turns
+// it on while the film update runs beside the H update: there the lean kernel fits two waves
+// per SIMD where the general one fits one.  Alone, one launch is faster.
+void yee_shell_split_x(bool on);
 }  // namespace mp

@@ -347,6 +347,10 @@ struct ShellGeom {
     uint32_t sgx;                        // strip groups per box row in this layout: gx, or 0 when the
                                          // x strips have a launch of their own
     uint32_t nstrip, r_nj;               // groups of all x strips; ceil(2^32 / nj)
+    // the rest rows (every row outside the box's y or z range) split by x: the groups in the
+    // box's x range (no x-slab cell) and the x edges; nnox = 0 when that split is not made
+    uint32_t nrlo, nrmid, nnox, nedge;   // rows of the low z planes / of the box planes; groups
+    uint32_t r_ny, r_2my, r_ngi;         // ceil(2^32 / d) of ny, 2 my, ngi
 };
 __device__ __forceinline__ uint32_t udiv(uint32_t n, uint32_t d, uint32_t r) {
     uint32_t q = __umulhi(n, r);
@@ -392,10 +396,56 @@ __device__ __forceinline__ void strip_pos(const ShellGeom &sg, uint32_t t, uint3
     g = s < sg.gi0 ? s : s + sg.ngi;
 }
 
-// The group a lane of a vector kernel updates: either the flattened plane mapping above
-// (the whole volume, one z-plane per workgroup) or shell entry t.  in_range is false for
-// lanes past the end; they still load from valid addresses and take part in the shuffles.
+// Rest row r of the shell, in memory order: the mz low z planes (ny rows each), the my low
+// and my high rows of each box plane, the mz high z planes.
+__device__ __forceinline__ void rest_row(const ShellGeom &sg, uint32_t r, uint32_t &j, uint32_t &k) {
+    if (r < sg.nrlo) {
+        k = udiv(r, sg.ny, sg.r_ny);
+        j = r - k * sg.ny;
+    } else if (r < sg.nrlo + sg.nrmid) {
+        const uint32_t u = r - sg.nrlo, m2 = 2 * sg.my;
+        const uint32_t kk = udiv(u, m2, sg.r_2my), jj = u - kk * m2;
+        k = sg.mz + kk;
+        j = jj < sg.my ? jj : sg.ny - m2 + jj;
+    } else {
+        const uint32_t u = r - sg.nrlo - sg.nrmid, kk = udiv(u, sg.ny, sg.r_ny);
+        j = u - kk * sg.ny;
+        k = sg.nz - sg.mz + kk;
+    }
+}
+
+// The group a lane of a vector kernel updates; in_range is false for lanes past the end.
 struct LaneGroup { int64_t i0, j, k; bool in_range; };
+
+// Shell modes of the vector kernels: 0 the shell layout (or the plane mapping), 1 the x strips
+// of the box rows (no y or z slab), 2 the box's x range of the rest rows (no x slab, no x
+// boundary), 3 the x edges of the rest rows.
+enum { SH_ALL = 0, SH_STRIP = 1, SH_NOX = 2, SH_EDGE = 3 };
+template <int MODE>
+__device__ __forceinline__ LaneGroup mode_group(const ShellGeom &sg, int tyv) {
+    const uint32_t t = blockIdx.x * (uint32_t)(64 * tyv) + threadIdx.y * 64 + threadIdx.x;
+    const uint32_t n = MODE == SH_STRIP ? sg.nstrip : (MODE == SH_NOX ? sg.nnox : sg.nedge);
+    LaneGroup r;
+    r.in_range = t < n;
+    const uint32_t tc = r.in_range ? t : 0u;
+    uint32_t g, j, k;
+    if (MODE == SH_STRIP) {
+        strip_pos(sg, tc, g, j, k);
+    } else {
+        const uint32_t w = MODE == SH_EDGE ? sg.gx : sg.ngi, rw = MODE == SH_EDGE ? sg.r_gx : sg.r_ngi;
+        const uint32_t row = udiv(tc, w, rw), s = tc - row * w;
+        g = MODE == SH_EDGE ? (s < sg.gi0 ? s : s + sg.ngi) : sg.gi0 + s;
+        rest_row(sg, row, j, k);
+    }
+    r.i0 = (int64_t)g * VX;
+    r.j = (int64_t)j;
+    r.k = (int64_t)k;
+    return r;
+}
+
+// The group of a lane in mode 0: either the flattened plane mapping above (the whole
+// volume, one z-plane per workgroup) or shell entry t.  Lanes past the end still load from
+// valid addresses and take part in the shuffles.
 __device__ __forceinline__ LaneGroup lane_group(const ShellGeom &sg, int64_t nx, int64_t ny, int64_t bpp, int tyv) {
     LaneGroup r;
     if (sg.n) {
@@ -429,10 +479,11 @@ __device__ __forceinline__ bool lane_holds(int64_t o, int delta) {
 
 // PP: the updated H goes to Hxo/Hyo/Hzo (the other buffer set of a ping-pong step) instead of
 // back to Hx/Hy/Hz; the whole group is stored, so a cell at i = nx-1 carries its old value over.
-// STRIP: the lanes take the x strips of the box rows (strip_pos), where no y or z slab term
-// exists: those terms and their registers drop out, and the kernel fits two waves per SIMD
-// beside the film update, where the general form fits one.
-template <typename P, int TYV, bool PP, bool STRIP = false>
+// MODE (SH_*): SH_STRIP takes the x strips of the box rows, where no y or z slab term exists,
+// and SH_NOX the box's x range of the rest rows, where no x slab term exists: those terms and
+// their registers drop out, and either kernel fits two waves per SIMD beside the film update,
+// where the general form fits one.  SH_EDGE takes the x edges of the rest rows.
+template <typename P, int TYV, bool PP, int MODE = SH_ALL>
 __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const float *__restrict__ Ex, const float *__restrict__ Ey, const float *__restrict__ Ez,
     float *__restrict__ Hx, float *__restrict__ Hy, float *__restrict__ Hz,
@@ -440,18 +491,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     CpmlArgs cp, int64_t nx, int64_t ny, int64_t nz, float dt_mu0, int64_t nbx, ShellGeom sg,
     float *__restrict__ Hxo, float *__restrict__ Hyo, float *__restrict__ Hzo) {
     __builtin_amdgcn_s_setprio(3);   // issue ahead of a film update running beside it
-    LaneGroup lg;
-    if (STRIP) {
-        const uint32_t t = blockIdx.x * (uint32_t)(64 * TYV) + threadIdx.y * 64 + threadIdx.x;
-        lg.in_range = t < sg.nstrip;
-        uint32_t g, j, k;
-        strip_pos(sg, lg.in_range ? t : 0u, g, j, k);
-        lg.i0 = (int64_t)g * VX;
-        lg.j = (int64_t)j;
-        lg.k = (int64_t)k;
-    } else {
-        lg = lane_group(sg, nx, ny, nbx, TYV);
-    }
+    const LaneGroup lg = MODE == SH_ALL ? lane_group(sg, nx, ny, nbx, TYV) : mode_group<MODE>(sg, TYV);
     // the whole wave takes part in the shuffles; out-of-range lanes only idle
     const bool act = lg.in_range && lg.j < ny - 1 && lg.k < nz - 1;
     const int64_t k = act ? lg.k : 0;
@@ -464,10 +504,10 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     const int64_t ic = act ? lg.i0 : 0;   // clamp addresses of idle lanes
     const int64_t jc = act ? lg.j : (int64_t)0;   // rows 0 and 1 exist (ny >= 2)
     int64_t jsy = 0;
-    const int ry = STRIP ? 0 : pml_slab(jc, ny, wy, &jsy);
+    const int ry = MODE == SH_STRIP ? 0 : pml_slab(jc, ny, wy, &jsy);
     const float4 idx4 = ld4(idcx + ic), xk4 = ld4(xk + ic);
     const float idy_ = idcy[jc], yk_ = yk[jc], ya_ = ya[jc], yb_ = yb[jc];
-    const bool xslab = wx > 0 && (ic < wx || ic + VX > nx - wx);
+    const bool xslab = MODE != SH_NOX && wx > 0 && (ic < wx || ic + VX > nx - wx);
     const int64_t o = ic + sy * jc + sz * k;
     const float4 ex_c = ld4(Ex + o), ey_c = ld4(Ey + o);
     const float4 ex_n = ld4(Ex + o + sz), ey_n = ld4(Ey + o + sz);
@@ -482,7 +522,7 @@ __global__ __launch_bounds__(64 * TYV) void yee_H_vec_kernel(
     }
     const float idz_ = idcz[k], zk_ = zk[k];
     int64_t jsz = 0;
-    const int rz = STRIP ? 0 : pml_slab(k, nz, wz, &jsz);
+    const int rz = MODE == SH_STRIP ? 0 : pml_slab(k, nz, wz, &jsz);
     const float za_ = rz ? za[k] : 0.f, zb_ = rz ? zb[k] : 0.f;
     // y-slab psi of Hx and Hz, z-slab psi of Hx and Hy (lo or hi slab array)
     const int64_t oy0 = ic + nx * (jsy + wy * k), oz0 = ic + nx * (jc + ny * jsz);
@@ -1190,8 +1230,23 @@ static ShellGeom shell_geom(const mp_yee *g, const Box &b, bool strips_apart = f
     sg.sgx = (uint32_t)sgx;
     sg.nstrip = (uint32_t)nstrip;
     sg.r_nj = recip(b.nj);
+    // the rest rows by x (strips apart only): every divisor at least 2, and the three parts
+    // with the box cover the volume
+    const int64_t nrlo = b.k0 * ny, nrmid = b.nk * 2 * b.j0, rows = 2 * nrlo + nrmid;
+    if (strips_apart && ny >= 2 && b.ngi >= 2 && b.j0 >= 1 && rows * G + nstrip + b.n() == G * ny * nz) {
+        sg.nrlo = (uint32_t)nrlo;
+        sg.nrmid = (uint32_t)nrmid;
+        sg.nnox = (uint32_t)(rows * b.ngi);
+        sg.nedge = (uint32_t)(rows * gx);
+        sg.r_ny = recip(ny);
+        sg.r_2my = recip(2 * b.j0);
+        sg.r_ngi = recip(b.ngi);
+    }
     return sg;
 }
+
+static bool g_split_x = false;
+void yee_shell_split_x(bool on) { g_split_x = on; }
 
 // pp (ping-pong step): only the CPML-shell launch, from g's E and H into pp's H; the interior
 // is then updated by the fused kernel.
@@ -1227,13 +1282,15 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         return MP_OK;
     }
     KernelTimer t("yee_H");
-    // the x strips of the box rows in a launch of their own, the rest of the shell in another;
-    // strips, interior, rest measured 1.486 ms per 512 x 512 x 256 pump step against 1.488-1.491
-    // for the other orders
+    // the x strips of the box rows in a launch of their own, the rest of the shell in another
+    // (split by x beside the film update where the layout allows: the box's x range without
+    // x-slab terms, then the x edges; 512 x 512 x 256 pump step 1.534 -> 1.518 ms, while alone
+    // the split costs the 384^3 stencil 1.045 -> 1.052 ms); strips, interior, rest measured
+    // 1.486 ms per 512 x 512 x 256 pump step against 1.488-1.491 for the other orders
     const ShellGeom ss = shell_geom(g, b, true);
     const ShellGeom &sr = ss.n ? ss : sg;
     if (ss.n) {
-        yee_H_vec_kernel<P, TY, false, true><<<cdiv(ss.nstrip, 64 * TY), block, 0, stream()>>>(
+        yee_H_vec_kernel<P, TY, false, SH_STRIP><<<cdiv(ss.nstrip, 64 * TY), block, 0, stream()>>>(
             (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
             (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
             (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, ss, nullptr, nullptr,
@@ -1247,11 +1304,19 @@ static int launch_H_vec_t(const mp_yee *g, const mp_cpml *c, double dt, double m
         (const float *)c->inv_kappa[2], nx, ny, (uint32_t)b.gi0, (uint32_t)b.ngi, (uint32_t)b.j0,
         (uint32_t)(b.ngi * b.nj), (uint32_t)b.k0, (uint32_t)b.n(), dt_mu0);
     MP_LAUNCH_CHECK("yee_H_int_kernel");
-    yee_H_vec_kernel<P, TY, false><<<cdiv(sr.n, 64 * TY), block, 0, stream()>>>(
-        (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,
-        (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,
-        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, sr, nullptr, nullptr,
-        nullptr);
+#define H_SHELL_ARGS(geom)                                                                                  \
+    (const float *)g->ex, (const float *)g->ey, (const float *)g->ez, (float *)g->hx, (float *)g->hy,          \
+        (float *)g->hz, (const float *)g->inv_d_cell_x, (const float *)g->inv_d_cell_y,                        \
+        (const float *)g->inv_d_cell_z, make_cpml_args(c, true), nx, ny, nz, dt_mu0, 1, geom, nullptr, nullptr, \
+        nullptr
+    if (g_split_x && ss.n && ss.nnox) {
+        yee_H_vec_kernel<P, TY, false, SH_NOX><<<cdiv(ss.nnox, 64 * TY), block, 0, stream()>>>(H_SHELL_ARGS(ss));
+        MP_LAUNCH_CHECK("yee_H_vec_kernel");
+        yee_H_vec_kernel<P, TY, false, SH_EDGE><<<cdiv(ss.nedge, 64 * TY), block, 0, stream()>>>(H_SHELL_ARGS(ss));
+    } else {
+        yee_H_vec_kernel<P, TY, false><<<cdiv(sr.n, 64 * TY), block, 0, stream()>>>(H_SHELL_ARGS(sr));
+    }
+#undef H_SHELL_ARGS
     MP_LAUNCH_CHECK("yee_H_vec_kernel");
     return MP_OK;
 }
@@ -1296,6 +1361,9 @@ static int launch_E_vec_t(const mp_yee *g, const mp_cpml *c, double dt, bool she
         else yee_E_int_kernel<false><<<cdiv(b.n(), 256), 256, 0, stream()>>>(E_ARGS, INT_ARGS);
         MP_LAUNCH_CHECK("yee_E_int_kernel");
     }
+    // the whole shell in one launch: split like the H shell (x strips, the box's x range of the
+    // rest rows, their x edges) it measured 682 against 666 us in the 512 x 512 x 256 pump step
+    // and 611 against 590 us in the 384^3 stencil
     const unsigned grid = cdiv(sg.n, 64 * TY);
     if (mi)
         yee_E_vec_kernel<P, TY, true, true><<<grid, block, 0, stream()>>>(

@@ -165,11 +165,11 @@ def source_digest(names) -> str:
 # dispatch; the factors are what the counters read for a plain 1 GiB device copy on gfx950).
 MEASURED_TRAFFIC = {
     ("coupled_pump_512x512x256", "plain", "index"): {
-        "digest": "8846214a23543467", "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md",
-        "bytes": {"yee_E": 4133403399, "yee_H": 3247397142}},
+        "digest": "6465cab51f2a17f7", "profile": "profiles/coupled_pump_512x512x256_plain_pmc.md",
+        "bytes": {"yee_E": 4133446194, "yee_H": 3247387641}},
     ("em_stencil_384^3", "plain", "index"): {
-        "digest": "8846214a23543467", "profile": "profiles/em_stencil_384_pmc.md",
-        "bytes": {"yee_E": 3635312156, "yee_H": 2865366375}},
+        "digest": "6465cab51f2a17f7", "profile": "profiles/em_stencil_384_pmc.md",
+        "bytes": {"yee_E": 3635337602, "yee_H": 2865344936}},
 }
 
 # FP64 operations of one film cell-substep of the 4TM + LLB update from SQ_INSTS_VALU_*_F64
