@@ -379,7 +379,10 @@ def test_split_yee_update_bitwise(shape, npml):
     """Grids whose H and E updates split into the interior-box launch and the CPML-shell
     launch, whose lanes decode their group from the shell layout (box edges on whole cache
     lines, shells of uneven and of one-cell width, rows that are not a multiple of the wave,
-    lanes past the end of the shell): every field and psi slab bit-identical to the oracle."""
+    lanes past the end of the shell): every field and psi slab bit-identical to the oracle.
+    From the second step on the film update of the step before runs beside the H update, and
+    the rest of the H shell then runs split by x (the box's x range without x-slab terms, then
+    the x edges), so both H shell forms are covered."""
     import driver
     from mphip import configs
     from mphip.state import FDTDState
